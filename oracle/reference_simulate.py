@@ -1,0 +1,393 @@
+"""Numpy restatement of the reference simulator (test oracle only).
+
+Follows ``superspinsim/generate_simulator.py`` step by step, with the same
+data flow as its CPU branch (numba ``prange`` loops):
+
+=====================================  =====================================
+step                                   reference
+=====================================  =====================================
+output times t_i = t0 + dt (i+1)       ``_calculate_time`` :158-159
+node times per fine division           ``_calculate_time_quadrature``
+                                       :161-167, call site :1956-1961
+sampler rows (zeroed, then filled)     ``sample_loop`` :253-261
+CF coefficient combination             ``_combine_coefficients`` :278-296
+A = (dt/F) sum_g c_g G_g               ``_calculate_differential`` :354-369
+rotating-frame A                       ``_calculate_differential_rotating``
+                                       :450-470, host precompute :1567-1677
+A / 4^R (+ I if not residual)          ``_scale_differential_basic`` :534-541
+2R squarings (residual 2X + X^2)       ``_square_superoperator`` :706-724,
+                                       ``_repeated_quartic...`` :882-897
+CF product, exponential J-1 first      ``_quadrature_combine_run``
+                                       :1259-1294 (residual L + R + LR)
+eigenbasis step exp(L0 h)              ``_apply_eig_*`` :902-1021 (GPU
+                                       bounds, :959)
+serial prefix product                  ``_basic_combine_run`` :1308-1321
+rotating / kernel / unitary sandwich   :1023-1187, 1394-1504, 2030-2105
+rho_t = rho0 + TE_t rho0               ``_apply_time_evolution`` :1325-1390
+flatten / unflatten                    :1524-1549, :2156-2191
+=====================================  =====================================
+
+Two places deliberately follow the *intended* semantics instead of a
+reference defect:
+
+* the CPU eigen-step iterates ``range(doubles_size)`` over the singles
+  (:985); the GPU kernel iterates ``singles_size`` (:959), which is used here;
+* the kernel-mode sandwich reads columns n'..n_d-1 outside the projection
+  arrays (:1078-1094 with bounds ``right.shape[0]``); here the full product
+  ``P (I + TE') P^T - I`` is formed, which equals the reference wherever its
+  reads are in bounds.
+"""
+
+import math
+
+import numpy as np
+
+from superspinsim_amd.quadratures import scheme
+
+_f64 = np.float64
+
+
+def _matmul_residual(left, right, residual):
+    """``L + R + L R`` (residual) or ``L R``."""
+    prod = left @ right
+    return left + right + prod if residual else prod
+
+
+def rotating_precompute(generators, doubles, singles, nodes, time_step,
+                        fine, residual):
+    """Node-wise interaction-picture generators and the eigen-step factors.
+
+    Restates ``generate_simulator.py:1567-1677``: for every node offset
+    tau_s = c_s dt / F the generators are conjugated by exp(-L0 tau_s) on
+    the rows and exp(L0 tau_s) on the columns, in the real 2x2/1x1 block
+    eigenbasis.  Returns ``(gens_rot (S, G, n, n), doubles_step (nd, 2),
+    singles_step (ns,))`` where the step factors are those of exp(L0 dt/F)
+    (``expm1`` based when ``residual``).
+    """
+    S = nodes.size
+    tz = np.empty(S + 1, dtype=_f64)
+    tz[:-1] = nodes * time_step / fine
+    tz[-1] = time_step / fine
+    nd = doubles.shape[0]
+    ns = singles.size
+    sf = np.empty((S + 1, ns), dtype=_f64)
+    sb = np.empty_like(sf)
+    df = np.empty((S + 1, nd, 2), dtype=_f64)
+    db = np.empty_like(df)
+    for k in range(S + 1 - int(residual)):
+        sf[k] = np.exp(singles * tz[k])
+        sb[k] = np.exp(-singles * tz[k])
+        att = np.exp(doubles[:, 0] * tz[k])
+        amp = np.exp(-doubles[:, 0] * tz[k])
+        sin = np.sin(doubles[:, 1] * tz[k])
+        cos = np.cos(doubles[:, 1] * tz[k])
+        df[k, :, 0] = att * cos
+        df[k, :, 1] = -att * sin
+        db[k, :, 0] = amp * cos
+        db[k, :, 1] = amp * sin
+    if residual:
+        sf[-1] = np.expm1(singles * tz[-1])
+        sin2 = -2 * (np.sin(doubles[:, 1] * tz[-1] / 2) ** 2)
+        sin1 = np.sin(doubles[:, 1] * tz[-1])
+        expmr = np.expm1(doubles[:, 0] * tz[-1])
+        df[-1, :, 0] = (1 + expmr) * sin2 + expmr
+        df[-1, :, 1] = -(1 + expmr) * sin1
+
+    rot = np.empty((S,) + generators.shape, dtype=_f64)
+    for s in range(S):
+        g = generators.copy()
+        for e in range(nd):
+            r0 = g[:, 2 * e, :].copy()
+            r1 = g[:, 2 * e + 1, :].copy()
+            g[:, 2 * e, :] = db[s, e, 0] * r0 - db[s, e, 1] * r1
+            g[:, 2 * e + 1, :] = db[s, e, 1] * r0 + db[s, e, 0] * r1
+        for e in range(ns):
+            g[:, 2 * nd + e, :] *= sb[s, e]
+        for e in range(nd):
+            c0 = g[:, :, 2 * e].copy()
+            c1 = g[:, :, 2 * e + 1].copy()
+            tr = df[s, e, 0] * c0 + df[s, e, 1] * c1
+            ti = df[s, e, 1] * c0 - df[s, e, 0] * c1
+            g[:, :, 2 * e] = tr
+            g[:, :, 2 * e + 1] = -ti
+        for e in range(ns):
+            g[:, :, 2 * nd + e] *= sf[s, e]
+        rot[s] = g
+    return rot, df[-1].copy(), sf[-1].copy()
+
+
+def eig_step(te, doubles_step, singles_step, residual):
+    """Left-multiply by exp(L0 h) in the eigenbasis (GPU semantics :945-967).
+
+    ``te`` has shape (..., n, m); rows (2y, 2y+1) are the doubles, rows
+    2 nd + y the singles.
+    """
+    out = te.copy()
+    nd = doubles_step.shape[0]
+    m = te.shape[-1]
+    for y in range(nd):
+        d0, d1 = doubles_step[y]
+        r = te[..., 2 * y, :]
+        i = te[..., 2 * y + 1, :]
+        if residual:
+            sr = (1 + d0) * r - d1 * i
+            si = d1 * r + (1 + d0) * i
+            if 2 * y < m:
+                sr[..., 2 * y] += d0
+                si[..., 2 * y] += d1
+            if 2 * y + 1 < m:
+                sr[..., 2 * y + 1] -= d1
+                si[..., 2 * y + 1] += d0
+        else:
+            sr = d0 * r - d1 * i
+            si = d1 * r + d0 * i
+        out[..., 2 * y, :] = sr
+        out[..., 2 * y + 1, :] = si
+    for y in range(singles_step.size):
+        row = 2 * nd + y
+        if residual:
+            v = (1 + singles_step[y]) * te[..., row, :]
+            if row < m:
+                v[..., row] += singles_step[y]
+        else:
+            v = singles_step[y] * te[..., row, :]
+        out[..., row, :] = v
+    return out
+
+
+def flatten(rho, vectorisation_map, use_hermitian, n_d):
+    hilbert = rho.shape[0]
+    flat = np.empty(n_d, dtype=_f64)
+    for k in range(n_d):
+        if use_hermitian:
+            y, x, c = vectorisation_map[k]
+        else:
+            c = k % 2
+            x = (k // 2) % hilbert
+            y = (k // 2) // hilbert
+        v = rho[y, x]
+        flat[k] = np.real(v) if c == 0 else np.imag(v)
+    return flat
+
+
+def unflatten(flat, vectorisation_map, use_hermitian, hilbert):
+    n_t, n_d = flat.shape
+    out = np.zeros((n_t, hilbert, hilbert, 2), dtype=_f64)
+    for k in range(n_d):
+        if use_hermitian:
+            y, x, c = vectorisation_map[k]
+        else:
+            c = k % 2
+            x = (k // 2) % hilbert
+            y = (k // 2) // hilbert
+        out[:, y, x, c] = flat[:, k]
+        if use_hermitian and y != x:
+            out[:, x, y, c] = -flat[:, k] if c else flat[:, k]
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def generate_simulator(sampler, generators, vectorisation_map=None,
+                       use_cuda=False, datatype=np.float64,
+                       use_residual=True, number_of_quartic_repeats=35,
+                       number_of_exponentials=2, number_of_fine_divisions=1,
+                       use_rotating=False, vectors_real=None,
+                       inv_vectors_real=None, doubles=None, singles=None,
+                       use_kernel=False, full_projection=None,
+                       image_projection=None, use_unitary=False,
+                       elimination=None, duplication=None,
+                       use_hermitian=True, verbose=False, use_cayley=False,
+                       backend="numpy", vectorised_sampler=None, threads=0):
+    """Reference-algorithm simulator factory.
+
+    ``backend="numpy"`` runs the loops below; ``backend="c"`` runs the same
+    algorithm in ``cpu_ref.c``.  ``vectorised_sampler(times) -> (len, G)``
+    may replace the per-node sampler calls when tabulating coefficients.
+    """
+    if use_cayley:
+        raise TypeError("Cayley not implemented")
+    R = int(number_of_quartic_repeats)
+    scale = 4.0 ** R
+    gens = np.asarray(generators).astype(_f64)
+    n = gens.shape[1]
+    if use_hermitian and vectorisation_map is None:
+        use_hermitian = False
+    if use_hermitian:
+        n_d = vectorisation_map.shape[0]
+    else:
+        n_d = 2 * (n // 2) ** 2 if use_unitary else n
+    F = number_of_fine_divisions
+    nodes, weights, _ = scheme(number_of_exponentials)
+    S = nodes.size
+    J = weights.shape[0]
+    if use_rotating:
+        vectors_real = np.array(vectors_real, dtype=_f64)
+        inv_vectors_real = np.array(inv_vectors_real, dtype=_f64)
+        doubles = np.array(doubles, dtype=_f64)
+        singles = np.array(singles, dtype=_f64)
+        if doubles.shape[0] == 0:
+            doubles = doubles.reshape((0, 2))
+
+    def _exponentials(A):
+        A = A / scale
+        if not use_residual:
+            A = A + np.eye(n)
+        X = A
+        for _ in range(2 * R):
+            X = 2 * X + X @ X if use_residual else X @ X
+        return X
+
+    def simulate(density_operator_initial, time_start, time_end, time_step):
+        N = int((time_end - time_start) / time_step)
+        hilbert = density_operator_initial.shape[0]
+        rho0 = flatten(np.asarray(density_operator_initial), vectorisation_map,
+                       use_hermitian, n_d)
+        h = time_step / F
+        if use_rotating:
+            gens_rot, d_step, s_step = rotating_precompute(
+                gens, doubles, singles, nodes, time_step, F, use_residual)
+
+        time = time_start + time_step * (np.arange(N) + 1)
+        G = gens.shape[0]
+        coef = node_coefficients(sampler, G, N, S, F, nodes, time_start,
+                                 time_step, vectorised_sampler)
+        if backend == "c":
+            from oracle.cpu_ref import propagate as c_propagate
+            te = c_propagate(gens_rot if use_rotating else gens, weights,
+                             coef, h, R, use_residual, use_rotating,
+                             d_step if use_rotating else None,
+                             s_step if use_rotating else None, threads)
+        else:
+            te = _propagate_numpy(
+                gens_rot if use_rotating else gens, weights, coef, h, n,
+                use_residual, use_rotating,
+                d_step if use_rotating else None,
+                s_step if use_rotating else None, _exponentials)
+
+        eye = np.eye(n)
+        if use_rotating:
+            full = te + eye if use_residual else te
+            te = vectors_real @ full @ inv_vectors_real
+            if use_residual:
+                te = te - eye
+        if use_kernel:
+            P = np.asarray(image_projection, dtype=_f64)
+            full = te + np.eye(te.shape[-1]) if use_residual else te
+            te = P @ full @ P.T
+            te = te + (np.eye(P.shape[0]) - P @ P.T)
+            if use_residual:
+                te = te - np.eye(P.shape[0])
+        if use_unitary:
+            te = _unitary_lift(te, n, use_residual)
+            full = te + np.eye(te.shape[-1]) if use_residual else te
+            te = np.asarray(elimination) @ full @ np.asarray(duplication)
+            if use_residual:
+                te = te - np.eye(te.shape[-1])
+
+        if use_residual:
+            flat = rho0[None] + te @ rho0
+        else:
+            flat = te @ rho0
+        density = unflatten(flat, vectorisation_map, use_hermitian, hilbert)
+        return time, density
+
+    return simulate
+
+
+def _unitary_lift(te, n, residual):
+    """Real interleaved ``conj(U_out) (x) U_in`` block (:1394-1425)."""
+    half = n // 2
+    N = te.shape[0]
+    size = half * half
+    out = np.zeros((N, 2 * size, 2 * size), dtype=_f64)
+    for y in range(size):
+        yo, yi = divmod(y, half)
+        for x in range(size):
+            xo, xi = divmod(x, half)
+            o_r = te[:, 2 * yo, 2 * xo]
+            o_i = -te[:, 2 * yo + 1, 2 * xo]
+            i_r = te[:, 2 * yi, 2 * xi]
+            i_i = te[:, 2 * yi + 1, 2 * xi]
+            sr = o_r * i_r - o_i * i_i
+            si = o_r * i_i + o_i * i_r
+            if residual:
+                sr = sr + (yo == xo) * i_r + (yi == xi) * o_r
+                si = si + (yo == xo) * i_i + (yi == xi) * o_i
+            out[:, 2 * y, 2 * x] = sr
+            out[:, 2 * y + 1, 2 * x + 1] = sr
+            out[:, 2 * y + 1, 2 * x] = -si
+            out[:, 2 * y, 2 * x + 1] = si
+    return out
+
+
+def node_coefficients(sampler, G, N, S, F, nodes, time_start, time_step,
+                      vectorised_sampler=None):
+    """Sampler rows at every node: array (F, N, S, G).
+
+    Node time of output interval i, fine division r, node s is
+    ``((t0 + r dt / F) + dt i) + dt c_s / F`` (:161-167, :1956-1961).
+    """
+    ts = np.empty((F, N, S), dtype=_f64)
+    for r in range(F):
+        offset = time_start + r * time_step / F
+        for i in range(N):
+            start = offset + time_step * i
+            for s in range(S):
+                ts[r, i, s] = start + time_step * nodes[s] / F
+    if vectorised_sampler is not None:
+        return np.asarray(vectorised_sampler(ts.ravel()),
+                          dtype=_f64).reshape(F, N, S, G)
+    coef = np.zeros((F * N * S, G), dtype=_f64)
+    flat = ts.ravel()
+    for k in range(flat.size):
+        sampler(flat[k], coef[k])
+    return coef.reshape(F, N, S, G)
+
+
+def _propagate_numpy(gens, weights, coef, h, n, residual, rotating, d_step,
+                     s_step, exponentials):
+    """Fine-division loop and serial prefix product (numpy)."""
+    F, N, S, G = coef.shape
+    J = weights.shape[0]
+    if residual:
+        te = np.zeros((N, n, n), dtype=_f64)
+    else:
+        te = np.broadcast_to(np.eye(n), (N, n, n)).copy()
+    for r in range(F):
+        c = coef[r]
+        A = np.zeros((N, J, n, n), dtype=_f64)
+        if rotating:
+            for j in range(J):
+                acc = np.zeros((N, n, n), dtype=_f64)
+                for s in range(S):
+                    for g in range(G):
+                        acc = acc + (gens[s, g][None]
+                                     * c[:, s, g][:, None, None]) \
+                            * (h * weights[j, s])
+                A[:, j] = acc
+        else:
+            for j in range(J):
+                cw = np.zeros((N, G), dtype=_f64)
+                for s in range(S):
+                    cw = cw + weights[j, s] * c[:, s, :]
+                acc = np.zeros((N, n, n), dtype=_f64)
+                for g in range(G):
+                    acc = acc + cw[:, g][:, None, None] * gens[g][None]
+                A[:, j] = h * acc
+        E = exponentials(A.reshape(N * J, n, n)).reshape(N, J, n, n)
+        for j in range(J - 1, -1, -1):
+            te = _matmul_residual(E[:, j], te, residual)
+        if rotating:
+            te = eig_step(te, d_step, s_step, residual)
+    for t in range(N - 1):
+        te[t + 1] = _matmul_residual(te[t + 1], te[t], residual)
+    return te
+
+
+def exact_step_count(time_start, time_end, time_step, fine):
+    """(N_out, total propagator steps) with the reference's floor."""
+    n_out = int((time_end - time_start) / time_step)
+    return n_out, n_out * fine
+
+
+__all__ = ["generate_simulator", "rotating_precompute", "eig_step",
+           "flatten", "unflatten", "exact_step_count", "math"]
