@@ -1,0 +1,214 @@
+"""Benchmark: propagator steps/s of the 7-level NV ODMR sweep, FP64.
+
+One bench step is one whole sweep: every instance (sweep point) advanced
+through all of its propagator steps (h = 20 ps, CF4:2) in one launch of the
+HIP propagator, with the fluorescence read out every output interval.
+Inputs (generators, waveforms, initial states) are resident in HBM before
+the timed region.
+
+Prints one JSON line (rank 0, N = 1 GPU).
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--config", default="odmr7",
+                   choices=["odmr7", "odmr7-n19", "odmr3"])
+    p.add_argument("--instances", type=int, default=None,
+                   help="sweep points (default: the workload's count)")
+    p.add_argument("--cpu-outputs", type=int, default=None,
+                   help="output intervals of the CPU-baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def workload(name, count):
+    from superspinsim_amd import configs
+    if name == "odmr7":
+        return configs.odmr7(count or 4096)
+    if name == "odmr7-n19":
+        return configs.odmr7(count or 4096, use_incoherent=True)
+    return configs.odmr3(count or 256)
+
+
+def cpu_baseline(wl, outputs):
+    """Reference algorithm (oracle/cpu_ref.c, OpenMP) on one sweep point.
+
+    Sample: the first ``outputs`` output intervals of the first sweep
+    point (``outputs * fine`` propagator steps), parallel over output
+    intervals like the reference's CPU path.
+    """
+    import numpy as np
+    from oracle.reference_simulate import generate_simulator as oracle_gen
+    w = wl.waveform(wl.frequencies[0])
+
+    def vec_sampler(ts):
+        out = np.zeros((ts.size, 5))
+        out[:, 0] = 1.0
+        out[:, 1] = w.segments[1][0][2] * np.cos(w.segments[1][0][3] * ts)
+        out[:, 4] = w.base[4]
+        return out
+
+    threads = min(len(os.sched_getaffinity(0)),
+                  int(os.environ.get("OMP_NUM_THREADS", "1024")))
+    sim = oracle_gen(None, wl.generators, wl.vmap, backend="c",
+                     number_of_exponentials=wl.number_of_exponentials,
+                     number_of_fine_divisions=wl.fine,
+                     vectorised_sampler=vec_sampler, threads=threads)
+    t_end = outputs * wl.time_step
+    sim(wl.rho0_matrix, 0.0, 2 * wl.time_step, wl.time_step)  # warm
+    t0 = time.perf_counter()
+    sim(wl.rho0_matrix, 0.0, t_end * (1 + 1e-12), wl.time_step)
+    elapsed = time.perf_counter() - t0
+    steps = outputs * wl.fine
+    return {"value": steps / elapsed, "unit": "propagator steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{outputs} output intervals x {wl.fine} steps of one "
+                      f"{wl.name} sweep point ({steps} steps, "
+                      f"{elapsed:.1f} s), reference algorithm: 2R=70 "
+                      "residual squarings, CF product, serial prefix "
+                      "product (oracle/cpu_ref.c)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; "
+                                   "--gpus > 1 is not supported"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from superspinsim_amd.propagator import device_tensor  # noqa: F401
+
+    wl = workload(args.config, args.instances)
+    sim = wl.simulator()
+    waves = wl.waveforms()
+    n_inst = len(waves)
+    print(f"workload {wl.name}: {n_inst} instances x "
+          f"{wl.steps_per_instance()} steps, n={wl.n}", file=sys.stderr,
+          flush=True)
+
+    def one_sweep():
+        return sim.run(waves, wl.rho0, wl.time_start, wl.time_end,
+                       wl.time_step, obs=wl.obs, read="obs")
+
+    # warmup (also builds device tables and the Taylor plan)
+    for _ in range(max(args.warmup, 1)):
+        out = one_sweep()
+    torch.cuda.synchronize()
+    plan = dict(sim.last_plan)
+
+    # Pre-pack so the timed region is launch + kernel only.
+    prop, active, m, s = sim.plan(waves, wl.time_step)
+    from superspinsim_amd.sweep import Waveform, pack_waveforms
+    sub = []
+    for w in waves:
+        nw = Waveform(len(active), w.base[active])
+        for k, g in enumerate(active):
+            nw.segments[k] = list(w.segments[g])
+        sub.append(nw)
+    d_waves, d_segs = pack_waveforms(sub, len(active))
+    from superspinsim_amd.propagator import device_tensor as dt_
+    d_rho = dt_(np.repeat(wl.rho0[None], n_inst, axis=0))
+    out = torch.empty((n_inst, wl.n_out, 1), dtype=torch.float64,
+                      device="cuda")
+
+    def launch():
+        prop.run(n_inst=n_inst, fine=wl.fine, n_out=wl.n_out,
+                 h=wl.time_step / wl.fine, t0=wl.time_start,
+                 dt=wl.time_step, rho0=d_rho, waves=d_waves,
+                 segments=d_segs, taylor=m, substeps=s, read="obs",
+                 obs=wl.obs, out=out)
+
+    launch()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    events = [(torch.cuda.Event(enable_timing=True),
+               torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for ev0, ev1 in events:
+        ev0.record(stream)
+        launch()
+        ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kernel_ms = [a.elapsed_time(b) for a, b in events]
+    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+
+    steps_total = n_inst * wl.steps_per_instance() * args.steps
+    value = steps_total / wall
+    flops_step = sim.step_flops(plan)
+    flops_launch = flops_step * n_inst * wl.steps_per_instance()
+    achieved = flops_launch / avg_kernel_s / 1e12
+    fluor = out[:, :, 0].cpu().numpy()
+    ok = bool(np.all(np.isfinite(fluor)))
+
+    result = {
+        "metric": "propagator steps/s, 7-level NV ODMR FP64"
+        if args.config.startswith("odmr7") else
+        "propagator steps/s, 3-level NV ODMR FP64",
+        "value": value,
+        "unit": "propagator steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name,
+            "instances": n_inst,
+            "superoperator_n": wl.n,
+            "steps_per_instance": wl.steps_per_instance(),
+            "h_s": wl.time_step / wl.fine,
+            "scheme": "CF4:2",
+            "readout": "fluorescence every output interval",
+            "taylor_degree": plan["taylor"],
+            "substeps": plan["substeps"],
+            "ell_slots": plan["slots"],
+            "nnz": plan["nnz"],
+            "parallelism": "one instance per wavefront",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": "propagate_kernel",
+            "kernel_ms": avg_kernel_s * 1e3,
+            "flops_per_step": flops_step,
+        },
+        "finite": ok,
+    }
+    if not args.no_cpu:
+        outputs = args.cpu_outputs or (128 if wl.n > 20 else 999)
+        result["cpu_baseline"] = cpu_baseline(wl, outputs)
+        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

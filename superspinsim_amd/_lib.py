@@ -1,0 +1,96 @@
+"""ctypes binding of ``libssmx.so`` (the HIP propagator library).
+
+The library is built in-tree by ``superspinsim_amd.build.build_library`` (or
+``__graft_entry__.build()``) into ``superspinsim_amd/lib/libssmx.so``.  There
+is no fallback: every entry point of the package that simulates raises when
+the library or a HIP device is missing.
+"""
+
+import ctypes
+import os
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libssmx.so")
+
+SSMX_ROTATING = 1
+SSMX_COEF_WAVE = 2
+SSMX_READ_FULL = 4
+SSMX_READ_INTEGRAL = 8
+
+MAX_N = 64
+MAX_GEN = 8
+MAX_EXP = 6
+MAX_NODE = 3
+MAX_OBS = 8
+MAX_TAYLOR = 40
+
+
+class PropagateDesc(ctypes.Structure):
+    """Mirror of ``ssmx_propagate_desc`` (include/ssmx.h)."""
+    _fields_ = [
+        ("n", ctypes.c_int32), ("n_inst", ctypes.c_int32),
+        ("n_gen", ctypes.c_int32), ("slots", ctypes.c_int32),
+        ("n_exp", ctypes.c_int32), ("n_node", ctypes.c_int32),
+        ("fine", ctypes.c_int32), ("n_out", ctypes.c_int32),
+        ("taylor", ctypes.c_int32), ("substeps", ctypes.c_int32),
+        ("n_obs", ctypes.c_int32), ("flags", ctypes.c_int32),
+        ("first_out", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("coef_inst_stride", ctypes.c_int64),
+        ("h", ctypes.c_double), ("t0", ctypes.c_double),
+        ("dt", ctypes.c_double),
+    ]
+
+
+class Segment(ctypes.Structure):
+    """Mirror of ``ssmx_segment``."""
+    _fields_ = [("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("amp", ctypes.c_double), ("omega", ctypes.c_double),
+                ("phase", ctypes.c_double)]
+
+
+class Wave(ctypes.Structure):
+    """Mirror of ``ssmx_wave``."""
+    _fields_ = [("base", ctypes.c_double * MAX_GEN),
+                ("seg_begin", ctypes.c_int32 * MAX_GEN),
+                ("seg_count", ctypes.c_int32 * MAX_GEN)]
+
+
+#: every symbol include/ssmx.h declares
+EXPORTS = ("ssmx_propagate", "ssmx_last_error", "ssmx_version",
+           "ssmx_device_count")
+
+_lib = None
+
+
+class SsmxError(RuntimeError):
+    """A library call returned a negative code."""
+
+
+def load():
+    """Load ``libssmx.so`` once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} not found: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.ssmx_propagate.argtypes = [ctypes.POINTER(PropagateDesc)] + [vp] * 13
+    lib.ssmx_propagate.restype = ctypes.c_int
+    lib.ssmx_last_error.argtypes = []
+    lib.ssmx_last_error.restype = ctypes.c_char_p
+    lib.ssmx_version.argtypes = []
+    lib.ssmx_version.restype = ctypes.c_int
+    lib.ssmx_device_count.argtypes = []
+    lib.ssmx_device_count.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().ssmx_last_error().decode(errors="replace")
+        raise SsmxError(f"ssmx error {rc}: {msg}")
+    return rc

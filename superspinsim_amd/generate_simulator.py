@@ -1,0 +1,252 @@
+"""Drop-in ``generate_simulator`` running on the MI355X propagator.
+
+Same signature, defaults and return values as
+``superspinsim/generate_simulator.py:17-47`` / ``simulate`` (:1508-2202):
+
+    simulate = generate_simulator(sampler, generators, vectorisation_map, ...)
+    time, density = simulate(density_operator_initial, t0, t1, dt)
+
+``time`` is (N_out,) float64 with ``N_out = int((t1 - t0) / dt)`` and
+``time[i] = t0 + dt (i + 1)``; ``density`` is (N_out, N_h, N_h) complex128.
+
+What runs where:
+
+* host (numpy): flattening of rho0 through ``vectorisation_map``, the
+  rotating-frame tables, tabulation of the user's ``sampler`` at every node
+  time (``sampler(t, row)`` writes a zeroed row in place, as the reference's
+  ``sample_kernel`` does), and the final basis changes / unflattening;
+* device (``libssmx.so`` via ctypes): every propagator step of every output
+  interval, in one kernel launch.
+
+The reference builds each interval's operator by 2R residual squarings and a
+serial prefix product; here the state is propagated with a Taylor action of
+the same exponentials (degree chosen per call from a norm bound, truncation
+error below 1e-17 per step), which equals the reference in exact
+arithmetic.  ``use_residual=False`` selects the same exponentials; the
+reference's non-residual rounding loss (``1 + A / 4^R`` rounding the diagonal
+to 1) is not reproduced.
+"""
+
+import numpy as np
+
+from superspinsim_amd import quadratures
+from superspinsim_amd.propagator import (
+    Propagator, device_tensor, eig_lane_factors, rotating_tables,
+    taylor_plan)
+
+_f64 = np.float64
+
+
+class UnsupportedSchemeError(NameError, ValueError):
+    """``number_of_exponentials`` is not 1, 2, 3, 5 or 6.
+
+    The reference raises ``NameError`` for these (no table is selected,
+    generate_simulator.py:116-140); this subclass keeps that type.
+    """
+
+
+def _flatten(rho, vmap, use_hermitian, n_d):
+    hilbert = rho.shape[0]
+    k = np.arange(n_d)
+    if use_hermitian:
+        ys, xs, cs = vmap[:, 0], vmap[:, 1], vmap[:, 2]
+    else:
+        cs = k % 2
+        xs = (k // 2) % hilbert
+        ys = (k // 2) // hilbert
+    vals = np.asarray(rho)[ys, xs]
+    return np.where(cs == 0, np.real(vals), np.imag(vals)).astype(_f64)
+
+
+def _unflatten(flat, vmap, use_hermitian, hilbert):
+    n_t, n_d = flat.shape
+    k = np.arange(n_d)
+    if use_hermitian:
+        ys, xs, cs = vmap[:, 0], vmap[:, 1], vmap[:, 2]
+    else:
+        cs = k % 2
+        xs = (k // 2) % hilbert
+        ys = (k // 2) // hilbert
+    out = np.zeros((n_t, hilbert, hilbert, 2), dtype=_f64)
+    out[:, ys, xs, cs] = flat
+    if use_hermitian:
+        off = ys != xs
+        sign = np.where(cs[off] == 1, -1.0, 1.0)
+        out[:, xs[off], ys[off], cs[off]] = flat[:, off] * sign
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def tabulate(sampler, n_gen, n_out, fine, nodes, time_start, time_step):
+    """Sampler rows at the reference node times: array (N_out, F, S, G).
+
+    Node time of interval i, fine division r, node s is
+    ``((t0 + (r dt) / F) + dt i) + (dt c_s) / F``
+    (generate_simulator.py:161-167 with the call at :1956-1961).
+    """
+    S = nodes.size
+    r = np.arange(fine, dtype=_f64)
+    offset = time_start + (r * time_step) / fine            # (F,)
+    i = np.arange(n_out, dtype=_f64)
+    start = offset[None, :] + time_step * i[:, None]        # (N, F)
+    ts = start[:, :, None] + (time_step * nodes) / fine     # (N, F, S)
+    flat = ts.ravel()
+    coef = np.zeros((flat.size, n_gen), dtype=_f64)
+    for k in range(flat.size):
+        sampler(flat[k], coef[k])
+    return coef.reshape(n_out, fine, S, n_gen)
+
+
+def generate_simulator(
+        sampler, generators, vectorisation_map=None, use_cuda=True,
+        datatype=np.float64, use_residual=True, number_of_quartic_repeats=35,
+        number_of_exponentials=2, number_of_fine_divisions=1,
+        use_rotating=False, vectors_real=None, inv_vectors_real=None,
+        doubles=None, singles=None, use_kernel=False, full_projection=None,
+        image_projection=None, use_unitary=False, elimination=None,
+        duplication=None, use_hermitian=True, verbose=False,
+        use_cayley=False, tolerance=1e-17):
+    """Build ``simulate(rho0, t0, t1, dt)`` for one system (see module doc).
+
+    ``number_of_quartic_repeats`` is accepted for compatibility: the
+    exponentials are evaluated to ``tolerance`` instead of by 2R
+    squarings.  ``use_cuda`` is ignored (the HIP device is always used).
+    """
+    if use_cayley:
+        raise TypeError("Cayley not implemented")
+    if np.dtype(datatype) != np.float64:
+        raise NotImplementedError("only float64 is supported")
+    if number_of_exponentials not in quadratures.SCHEMES:
+        raise UnsupportedSchemeError(
+            f"number_of_exponentials={number_of_exponentials} has no CF "
+            "table (valid: 1, 2, 3, 5, 6)")
+    nodes, weights, label = quadratures.scheme(number_of_exponentials)
+    if verbose:
+        print(f"Using the {label} method with {nodes.size} nodes.")
+
+    gens = np.asarray(generators).astype(_f64)
+    n = gens.shape[1]
+    if use_hermitian and vectorisation_map is None:
+        use_hermitian = False
+    if use_hermitian:
+        vmap = np.asarray(vectorisation_map, dtype=np.int64)
+        n_d = vmap.shape[0]
+    else:
+        vmap = None
+        n_d = 2 * (n // 2) ** 2 if use_unitary else n
+    F = int(number_of_fine_divisions)
+    if F < 1:
+        raise ValueError("number_of_fine_divisions must be >= 1")
+
+    if use_rotating:
+        vectors_real = np.array(vectors_real, dtype=_f64)
+        inv_vectors_real = np.array(inv_vectors_real, dtype=_f64)
+        doubles = np.array(doubles, dtype=_f64)
+        singles = np.array(singles, dtype=_f64)
+        if doubles.shape[0] == 0:
+            doubles = doubles.reshape((0, 2))
+    if use_kernel:
+        image_projection = np.array(image_projection, dtype=_f64)
+    if use_unitary:
+        elimination = np.array(elimination, dtype=_f64)
+        duplication = np.array(duplication, dtype=_f64)
+
+    def simulate(density_operator_initial, time_start, time_end, time_step):
+        torch = __import__("torch")
+        N = int((time_end - time_start) / time_step)
+        rho = np.asarray(density_operator_initial)
+        hilbert = rho.shape[0]
+        time = time_start + time_step * (np.arange(N) + 1)
+        if N <= 0:
+            return time, np.zeros((0, hilbert, hilbert), dtype=complex)
+        h = time_step / F
+        rho_flat = _flatten(rho, vmap, use_hermitian, n_d)
+
+        coef = tabulate(sampler, gens.shape[0], N, F, nodes, time_start,
+                        time_step)
+        active = np.nonzero(np.any(coef != 0, axis=(0, 1, 2)))[0]
+        if active.size == 0:
+            active = np.array([0])
+        coef = np.ascontiguousarray(coef[..., active])
+        cmax = np.abs(coef).max(axis=(0, 1))                 # (S, K)
+
+        if use_rotating:
+            mats, d_step, s_step = rotating_tables(
+                gens, doubles, singles, nodes, time_step, F, use_residual)
+            mats = mats[:, active]
+            eig = eig_lane_factors(d_step, s_step, n, use_residual)
+        else:
+            mats = gens[active]
+            eig = None
+        prop = Propagator(mats, weights, nodes, rotating=use_rotating,
+                          eig=eig)
+        m, s = taylor_plan(prop.norm_bound(cmax, h), tol=tolerance)
+        if verbose:
+            print(f"Taylor degree {m}, {s} substeps, {prop.ell.slots} ELL "
+                  f"slots, nnz {prop.ell.nnz}")
+
+        if use_unitary:
+            # Propagate every column of the identity: the unitary's real
+            # interleaved form at each output (operator mode).
+            x0 = np.eye(n, dtype=_f64)
+        else:
+            x0 = rho_flat[None, :] if not use_kernel else \
+                (image_projection.T @ rho_flat)[None, :]
+            if use_rotating:
+                x0 = x0 @ inv_vectors_real.T
+        n_inst = x0.shape[0]
+        d_rho = device_tensor(x0)
+        d_coef = device_tensor(coef)
+        out = prop.run(n_inst=n_inst, fine=F, n_out=N, h=h, t0=time_start,
+                       dt=time_step, rho0=d_rho, coef=d_coef,
+                       coef_inst_stride=0, taylor=m, substeps=s,
+                       read="full")
+        torch.cuda.synchronize()
+        states = out.cpu().numpy()                           # (I, N, n)
+
+        if use_unitary:
+            return time, _unitary_density(states, n, rho_flat, elimination,
+                                          duplication, vmap, use_hermitian,
+                                          hilbert, use_rotating,
+                                          vectors_real, inv_vectors_real)
+        x = states[0]                                        # (N, n)
+        if use_rotating:
+            x = x @ vectors_real.T
+        if use_kernel:
+            base = rho_flat - image_projection @ (image_projection.T
+                                                  @ rho_flat)
+            x = x @ image_projection.T + base[None, :]
+        return time, _unflatten(x, vmap, use_hermitian, hilbert)
+
+    return simulate
+
+
+def _unitary_density(columns, n, rho_flat, elimination, duplication, vmap,
+                     use_hermitian, hilbert, use_rotating, vectors_real,
+                     inv_vectors_real):
+    """Lift propagated unitary columns to rho(t) (:1394-1504, 2074-2116).
+
+    ``columns[k, t]`` is the propagator at output t applied to unit vector
+    k, i.e. column k of the real interleaved unitary form.  The lift follows
+    the reference: element (y, x) of the superoperator, with
+    y = y_out * N + y_in, is conj(U[y_out, x_out]) U[y_in, x_in] in the real
+    2x2 form [[r, i], [-i, r]], then rho_t = E (lift) D rho0.
+    """
+    te = np.transpose(columns, (1, 2, 0))                    # (N_out, n, n)
+    if use_rotating:
+        te = vectors_real @ te @ inv_vectors_real
+    half = n // 2
+    ur = te[:, 0::2, 0::2]
+    ui = te[:, 1::2, 0::2]
+    u = ur + 1j * ui                                         # (N_out, N, N)
+    lifted = np.einsum("tac,tbd->tabcd", np.conj(u), u).reshape(
+        te.shape[0], half * half, half * half)
+    lr, li = lifted.real, lifted.imag
+    size = half * half
+    full = np.zeros((te.shape[0], 2 * size, 2 * size), dtype=_f64)
+    full[:, 0::2, 0::2] = lr
+    full[:, 1::2, 1::2] = lr
+    full[:, 1::2, 0::2] = -li
+    full[:, 0::2, 1::2] = li
+    flat = np.einsum("ij,tjk,kl,l->ti", elimination, full, duplication,
+                     rho_flat)
+    return _unflatten(flat, vmap, use_hermitian, hilbert)

@@ -1,0 +1,328 @@
+"""Device-side propagation: ELL packing, Taylor planning, kernel launch.
+
+``Propagator`` owns the device copies of a generator stack (HBM, via torch
+tensors on the HIP device) and launches ``ssmx_propagate`` for any number of
+instances.  It is the layer both the drop-in ``generate_simulator`` and the
+batched sweep API (``superspinsim_amd.sweep``) call.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from superspinsim_amd import _lib
+
+_f64 = np.float64
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "superspinsim_amd needs a HIP device (torch.cuda.is_available() "
+            "is False); there is no CPU fallback")
+    return torch
+
+
+def device_tensor(array, dtype=None):
+    torch = _torch()
+    arr = np.ascontiguousarray(array)
+    t = torch.from_numpy(arr)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda")
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------------------
+# ELL packing
+# ---------------------------------------------------------------------------
+
+class EllStack:
+    """Generators packed in one ELL pattern (the union of their non-zeros).
+
+    ``mats`` has shape (M, n, n); ``col`` is int32 (slots, 64) and ``val``
+    float64 (M, slots, 64), with rows >= n and padded slots set to
+    column 0 / value 0.
+    """
+
+    def __init__(self, mats):
+        mats = np.asarray(mats, dtype=_f64)
+        if mats.ndim != 3 or mats.shape[1] != mats.shape[2]:
+            raise ValueError(f"generator stack must be (M, n, n), got "
+                             f"{mats.shape}")
+        M, n, _ = mats.shape
+        if n > _lib.MAX_N:
+            raise ValueError(f"superoperator dimension {n} > {_lib.MAX_N}")
+        union = np.any(mats != 0, axis=0)
+        rows = [np.nonzero(union[r])[0] for r in range(n)]
+        slots = max(1, max(len(c) for c in rows))
+        col = np.zeros((slots, 64), dtype=np.int32)
+        val = np.zeros((M, slots, 64), dtype=_f64)
+        for r in range(n):
+            for k, c in enumerate(rows[r]):
+                col[k, r] = c
+                val[:, k, r] = mats[:, r, c]
+        self.n = n
+        self.slots = slots
+        self.col = col
+        self.val = val
+        self.nnz = int(union.sum())
+        self.norms = np.abs(mats).sum(axis=1).max(axis=1)  # 1-norms
+
+
+# ---------------------------------------------------------------------------
+# Taylor degree and substeps
+# ---------------------------------------------------------------------------
+
+def taylor_remainder(x, m):
+    """Bound on sum_{k>m} x^k / k! for x >= 0."""
+    if x == 0:
+        return 0.0
+    if x >= m + 2:
+        return math.inf
+    return x ** (m + 1) / math.factorial(m + 1) / (1 - x / (m + 2))
+
+
+def taylor_plan(norm, tol=1e-17, max_taylor=_lib.MAX_TAYLOR):
+    """Cheapest (m, s) with s * remainder(norm / s, m) <= tol.
+
+    ``norm`` bounds the 1-norm of every exponent A; the action
+    exp(A) x = (exp(A/s))^s x is evaluated with degree-m Taylor polynomials.
+    Cost is m * s sparse products; ties prefer fewer substeps.
+    """
+    best = None
+    s = 1
+    while s <= 1024:
+        x = norm / s
+        for m in range(1, max_taylor + 1):
+            if s * taylor_remainder(x, m) <= tol:
+                if best is None or m * s < best[0] * best[1]:
+                    best = (m, s)
+                break
+        s *= 2
+    if best is None:
+        raise ValueError(f"no Taylor plan reaches tol={tol} for "
+                         f"norm={norm}")
+    return best
+
+
+# ---------------------------------------------------------------------------
+# Rotating-frame precompute (restates generate_simulator.py:1567-1677)
+# ---------------------------------------------------------------------------
+
+def rotating_tables(generators, doubles, singles, nodes, time_step, fine,
+                    residual=True):
+    """Interaction-picture generators per node and exp(L0 h) step factors.
+
+    For node offset tau_s = (c_s dt) / F the generators are conjugated
+    ``exp(-L0 tau) G exp(L0 tau)`` in the real block eigenbasis of L0
+    (2x2 blocks for eigenvalue pairs a +- ib, 1x1 for real eigenvalues).
+    Returns ``(gens_rot (S, G, n, n), d_step (nd, 2), s_step (ns,))`` where
+    the step factors describe exp(L0 h) - I when ``residual`` (the
+    reference's expm1 forms, :1617-1624) and exp(L0 h) otherwise.
+    """
+    generators = np.asarray(generators, dtype=_f64)
+    nd = doubles.shape[0]
+    ns = singles.size
+    tz = nodes * time_step / fine
+    h = time_step / fine
+    a = doubles[:, 0] if nd else np.zeros(0)
+    b = doubles[:, 1] if nd else np.zeros(0)
+    out = np.empty((nodes.size,) + generators.shape, dtype=_f64)
+    for s, tau in enumerate(tz):
+        amp = np.exp(-a * tau)
+        att = np.exp(a * tau)
+        cos = np.cos(b * tau)
+        sin = np.sin(b * tau)
+        db0, db1 = amp * cos, amp * sin
+        df0, df1 = att * cos, -att * sin
+        g = generators.copy()
+        if nd:
+            r0 = g[:, 0:2 * nd:2, :].copy()
+            r1 = g[:, 1:2 * nd:2, :].copy()
+            g[:, 0:2 * nd:2, :] = db0[None, :, None] * r0 \
+                - db1[None, :, None] * r1
+            g[:, 1:2 * nd:2, :] = db1[None, :, None] * r0 \
+                + db0[None, :, None] * r1
+        if ns:
+            g[:, 2 * nd:2 * nd + ns, :] *= np.exp(-singles * tau)[None, :,
+                                                                   None]
+        if nd:
+            c0 = g[:, :, 0:2 * nd:2].copy()
+            c1 = g[:, :, 1:2 * nd:2].copy()
+            tr = df0[None, None, :] * c0 + df1[None, None, :] * c1
+            ti = df1[None, None, :] * c0 - df0[None, None, :] * c1
+            g[:, :, 0:2 * nd:2] = tr
+            g[:, :, 1:2 * nd:2] = -ti
+        if ns:
+            g[:, :, 2 * nd:2 * nd + ns] *= np.exp(singles * tau)[None, None,
+                                                                  :]
+        out[s] = g
+    if residual:
+        s_step = np.expm1(singles * h)
+        sin2 = -2 * (np.sin(b * h / 2) ** 2)
+        sin1 = np.sin(b * h)
+        expmr = np.expm1(a * h)
+        d_step = np.stack([(1 + expmr) * sin2 + expmr, -(1 + expmr) * sin1],
+                          axis=1) if nd else np.zeros((0, 2))
+    else:
+        s_step = np.exp(singles * h)
+        att = np.exp(a * h)
+        d_step = np.stack([att * np.cos(b * h), -att * np.sin(b * h)],
+                          axis=1) if nd else np.zeros((0, 2))
+    return out, d_step, s_step
+
+
+def eig_lane_factors(d_step, s_step, n, residual=True):
+    """Per-row (own, partner) factors so that x <- x + own x + part x[r^1].
+
+    With ``residual`` the step factors are exp(L0 h) - I (expm1 forms);
+    otherwise they are exp(L0 h) and 1 is subtracted here.
+    """
+    own = np.zeros(64, dtype=_f64)
+    part = np.zeros(64, dtype=_f64)
+    nd = d_step.shape[0]
+    shift = 0.0 if residual else 1.0
+    for y in range(nd):
+        d0, d1 = d_step[y]
+        own[2 * y] = d0 - shift
+        part[2 * y] = -d1
+        own[2 * y + 1] = d0 - shift
+        part[2 * y + 1] = d1
+    for y in range(s_step.size):
+        own[2 * nd + y] = s_step[y] - shift
+    if 2 * nd + s_step.size != n:
+        raise ValueError("eigenbasis size does not match the generators")
+    return np.concatenate([own, part])
+
+
+# ---------------------------------------------------------------------------
+# Launch
+# ---------------------------------------------------------------------------
+
+class Propagator:
+    """Device copy of one generator stack and its CF scheme.
+
+    ``mats`` is (K, n, n) (non-rotating) or (S, K, n, n) (rotating frame,
+    node-wise generators).  ``weights`` (J, S) and ``nodes`` (S,) are the CF
+    tables.  ``eig`` is the (128,) lane factor array of the rotating
+    eigen-step, or None.
+    """
+
+    def __init__(self, mats, weights, nodes, rotating=False, eig=None):
+        torch = _torch()
+        _lib.load()
+        mats = np.asarray(mats, dtype=_f64)
+        self.rotating = bool(rotating)
+        weights = np.asarray(weights, dtype=_f64)
+        nodes = np.asarray(nodes, dtype=_f64)
+        self.J, self.S = weights.shape
+        if nodes.size != self.S:
+            raise ValueError("nodes and weights disagree on S")
+        if self.J > _lib.MAX_EXP or self.S > _lib.MAX_NODE:
+            raise ValueError(f"CF scheme J={self.J}, S={self.S} beyond "
+                             "library limits")
+        if self.rotating:
+            if mats.ndim != 4 or mats.shape[0] != self.S:
+                raise ValueError("rotating generators must be (S, K, n, n)")
+            self.K = mats.shape[1]
+            flat = mats.reshape(-1, mats.shape[-2], mats.shape[-1])
+        else:
+            if mats.ndim != 3:
+                raise ValueError("generators must be (K, n, n)")
+            self.K = mats.shape[0]
+            flat = mats
+        if self.K > _lib.MAX_GEN:
+            raise ValueError(f"{self.K} generators > {_lib.MAX_GEN}")
+        self.ell = EllStack(flat)
+        self.n = self.ell.n
+        norms = self.ell.norms
+        self.gen_norms = norms.reshape(self.S, self.K) if self.rotating \
+            else norms
+        self.weights = weights
+        self.nodes = nodes
+        self.d_col = device_tensor(self.ell.col)
+        self.d_val = device_tensor(self.ell.val)
+        self.d_w = device_tensor(weights)
+        self.d_nodes = device_tensor(nodes)
+        self.d_eig = device_tensor(eig) if eig is not None else None
+        self.torch = torch
+
+    def norm_bound(self, coef_abs_max, h):
+        """Bound on ||A_j||_1 from per-node maxima |c[s, g]| (S, K)."""
+        c = np.asarray(coef_abs_max, dtype=_f64).reshape(self.S, self.K)
+        best = 0.0
+        for j in range(self.J):
+            if self.rotating:
+                v = np.sum(np.abs(h * self.weights[j][:, None]) * c
+                           * self.gen_norms)
+            else:
+                cw = np.abs(self.weights[j]) @ c
+                v = h * float(np.sum(cw * self.gen_norms))
+            best = max(best, float(v))
+        return best
+
+    def run(self, *, n_inst, fine, n_out, h, t0, dt, rho0, coef=None,
+            coef_inst_stride=0, waves=None, segments=None, taylor=None,
+            substeps=1, read="full", obs=None, obs_offset=None, out=None,
+            state_out=None, first_out=0, stream=None):
+        """Launch one propagation; returns the output tensor.
+
+        ``rho0`` (n_inst, n) device tensor.  Coefficients: ``coef`` device
+        table (..., n_out*fine*S*K) or ``waves``/``segments`` device buffers
+        of ``ssmx_wave``/``ssmx_segment`` bytes.  ``read`` is "full",
+        "obs" or "integral".
+        """
+        torch = self.torch
+        flags = _lib.SSMX_ROTATING if self.rotating else 0
+        n_obs = 0
+        if read == "full":
+            flags |= _lib.SSMX_READ_FULL
+            shape = (n_inst, n_out, self.n)
+        else:
+            if obs is None:
+                raise ValueError("obs readout needs obs rows")
+            n_obs = obs.shape[0]
+            if read == "integral":
+                flags |= _lib.SSMX_READ_INTEGRAL
+                shape = (n_inst, n_obs)
+            elif read == "obs":
+                shape = (n_inst, n_out, n_obs)
+            else:
+                raise ValueError(f"unknown readout {read!r}")
+        if waves is not None:
+            flags |= _lib.SSMX_COEF_WAVE
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device="cuda")
+        if taylor is None:
+            raise ValueError("taylor degree required")
+        desc = _lib.PropagateDesc(
+            n=self.n, n_inst=n_inst, n_gen=self.K, slots=self.ell.slots,
+            n_exp=self.J, n_node=self.S, fine=fine, n_out=n_out,
+            taylor=int(taylor), substeps=int(substeps), n_obs=max(n_obs, 1),
+            flags=flags, first_out=int(first_out), reserved=0,
+            coef_inst_stride=int(coef_inst_stride), h=float(h),
+            t0=float(t0), dt=float(dt))
+        obs_rows = None
+        if obs is not None:
+            rows = np.zeros((n_obs, 64), dtype=_f64)
+            rows[:, :self.n] = obs
+            obs_rows = device_tensor(rows)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        lib = _lib.load()
+        rc = lib.ssmx_propagate(
+            ctypes.byref(desc), _ptr(self.d_col), _ptr(self.d_val),
+            _ptr(self.d_w), _ptr(self.d_nodes),
+            _ptr(waves if waves is not None else coef), _ptr(segments),
+            _ptr(rho0), _ptr(self.d_eig), _ptr(obs_rows), _ptr(obs_offset),
+            _ptr(out), _ptr(state_out), ctypes.c_void_p(stream))
+        _lib.check(rc)
+        # keep the readout rows alive until the stream has consumed them
+        self._keep = obs_rows
+        return out
