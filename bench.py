@@ -35,6 +35,11 @@ def parse():
     p.add_argument("--cpu-outputs", type=int, default=None,
                    help="output intervals of the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--gather", default="lds", choices=["lds", "bpermute"],
+                   help="neighbour exchange of the sparse products")
+    p.add_argument("--outputs", type=int, default=None,
+                   help="truncate every instance to this many output "
+                        "intervals (profiling runs)")
     return p.parse_args()
 
 
@@ -97,6 +102,9 @@ def main():
     from superspinsim_amd.propagator import device_tensor  # noqa: F401
 
     wl = workload(args.config, args.instances)
+    if args.outputs:
+        wl.time_end = args.outputs * wl.time_step * (1 + 1e-12)
+        wl.n_out = args.outputs
     sim = wl.simulator()
     waves = wl.waveforms()
     n_inst = len(waves)
@@ -134,7 +142,7 @@ def main():
                  h=wl.time_step / wl.fine, t0=wl.time_start,
                  dt=wl.time_step, rho0=d_rho, waves=d_waves,
                  segments=d_segs, taylor=m, substeps=s, read="obs",
-                 obs=wl.obs, out=out)
+                 obs=wl.obs, out=out, gather=args.gather)
 
     launch()
     torch.cuda.synchronize()
@@ -189,6 +197,7 @@ def main():
             "ell_slots": plan["slots"],
             "nnz": plan["nnz"],
             "parallelism": "one instance per wavefront",
+            "gather": args.gather,
         },
         "roofline": {
             "bound": "mfma",

@@ -21,8 +21,9 @@
  * Data layout of a propagation problem (one launch advances n_inst
  * independent instances over n_out * fine propagator steps each):
  *  - Superoperators are n x n real matrices (n <= 64), stored in a shared
- *    ELL pattern: column index ell_col[slot * 64 + row] (int32, padded rows
- *    point at column 0 with value 0) and values
+ *    ELL pattern: column index ell_col[slot * 64 + row] (int32); slot 0 of
+ *    row r must be column r (the diagonal, value possibly 0); unused slots
+ *    point at column r with value 0; values
  *    ell_val[((node * n_gen + g) * slots + slot) * 64 + row]; node ranges
  *    over n_node for the rotating frame (node-wise interaction-picture
  *    generators) and is 0 otherwise.
@@ -56,13 +57,16 @@ extern "C" {
 #define SSMX_COEF_WAVE 2     /* coefficients from waveform descriptors */
 #define SSMX_READ_FULL 4     /* readout: whole state vector (n values) */
 #define SSMX_READ_INTEGRAL 8 /* readout: trapezoid integral of obs rows */
+#define SSMX_GATHER_LDS 16   /* exchange neighbour components through an
+                                LDS copy of the state (ds_read_b64) instead
+                                of ds_bpermute */
 
 /* limits */
 #define SSMX_MAX_N 64
 #define SSMX_MAX_GEN 8
 #define SSMX_MAX_EXP 6
 #define SSMX_MAX_NODE 3
-#define SSMX_MAX_OBS 8
+#define SSMX_MAX_OBS 2
 #define SSMX_MAX_TAYLOR 40
 
 typedef struct ssmx_propagate_desc {
@@ -76,11 +80,12 @@ typedef struct ssmx_propagate_desc {
     int32_t n_out;    /* output samples per instance */
     int32_t taylor;   /* Taylor degree m of the action exp(A) v, 1..40 */
     int32_t substeps; /* s: exp(A) v = (exp(A/s))^s v, power of two */
-    int32_t n_obs;    /* readout rows for obs readouts, 1..8 */
+    int32_t n_obs;    /* readout rows for obs readouts, 1..2 */
     int32_t flags;    /* SSMX_* flags */
     int32_t first_out; /* global index of the first output interval of this
                           launch (node times count intervals from t0) */
-    int32_t reserved;
+    int32_t partner;  /* 1: ELL slot 1 of row r is column r ^ 1 for every
+                         row (exchanged by DPP instead of ds_bpermute) */
     int64_t coef_inst_stride; /* table mode: doubles between instances;
                                  0 = one table shared by all instances */
     double h;         /* propagator step dt / F (s) */
@@ -119,7 +124,7 @@ typedef struct ssmx_wave {
  * rho0       double [n_inst * n]
  * eig        rotating: double [2 * 64] per-row (own, partner) factors of
  *            exp(L0 h) - I; partner of row r is row r ^ 1
- * obs        double [n_obs * 64] readout rows (obs readouts)
+ * obs        double [n_obs * 64] readout rows (obs readouts, n_obs <= 2)
  * obs_offset double [n_inst * n_obs] added to each obs readout (or NULL)
  * out        READ_FULL: double [n_inst * n_out * n]
  *            obs: double [n_inst * n_out * n_obs]

@@ -16,12 +16,13 @@ SSMX_ROTATING = 1
 SSMX_COEF_WAVE = 2
 SSMX_READ_FULL = 4
 SSMX_READ_INTEGRAL = 8
+SSMX_GATHER_LDS = 16
 
 MAX_N = 64
 MAX_GEN = 8
 MAX_EXP = 6
 MAX_NODE = 3
-MAX_OBS = 8
+MAX_OBS = 2
 MAX_TAYLOR = 40
 
 
@@ -34,7 +35,7 @@ class PropagateDesc(ctypes.Structure):
         ("fine", ctypes.c_int32), ("n_out", ctypes.c_int32),
         ("taylor", ctypes.c_int32), ("substeps", ctypes.c_int32),
         ("n_obs", ctypes.c_int32), ("flags", ctypes.c_int32),
-        ("first_out", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("first_out", ctypes.c_int32), ("partner", ctypes.c_int32),
         ("coef_inst_stride", ctypes.c_int64),
         ("h", ctypes.c_double), ("t0", ctypes.c_double),
         ("dt", ctypes.c_double),

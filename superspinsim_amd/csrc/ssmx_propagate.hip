@@ -7,7 +7,7 @@
 // Per step of size h (reference: generate_simulator.py:1954-2015):
 //   for j = J-1 .. 0:                      (CF product order, :1259-1294)
 //     A_j = h sum_g c~_jg G_g, c~_jg = sum_s w[j,s] c[s,g]     (:278-448)
-//       or, rotating frame, A_j = sum_s sum_g (G~[s,g] c[s,g]) (h w[j,s])
+//       or, rotating frame, A_j = sum_s sum_g (c[s,g] h w[j,s]) G~[s,g]
 //                                                             (:450-532)
 //     x <- exp(A_j) x
 //   rotating frame: x <- exp(L0 h) x (2x2 / 1x1 blocks, :902-1021)
@@ -17,15 +17,17 @@
 // (optionally in 2^k substeps), which is exact to the same tolerance and
 // needs m sparse matrix-vector products instead of 70 dense products.
 //
-// Superoperators are stored in an ELL pattern shared by every generator and
-// instance: lane r keeps the column index and the assembled value of its
-// `slots` non-zeros in registers; neighbour components are fetched with
-// ds_bpermute (a register crossbar, no LDS bank traffic).  Generator values
-// are staged once per workgroup in LDS.
-//
-// Coefficients come from a host-tabulated table (drop-in path: the user's
-// Python sampler evaluated at the reference's node times) or from waveform
-// descriptors evaluated on the device (sweeps and protocols).
+// Data movement per Taylor term (the hot loop):
+//  - slot 0 of every ELL row is the diagonal: no exchange;
+//  - slot 1 is the partner column r ^ 1 (the Re/Im pair of a coherence, or
+//    a 2x2 eigen-block): exchanged with a DPP quad permutation (VALU);
+//  - slots 2.. are arbitrary columns: fetched with ds_bpermute (LDS
+//    crossbar, no bank conflicts).
+// Coefficients are evaluated lane-parallel in batches: lane e of a batch
+// computes the scalar factor of (step, exponential j, matrix m); the factors
+// are broadcast to the wavefront with readlane when the step runs.
+// Generator values live in VGPRs when they fit, else in LDS shared by the
+// workgroup's wavefronts.
 
 #include <hip/hip_runtime.h>
 
@@ -40,6 +42,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
+constexpr int kRegVals = 24;  // max matrix values per lane kept in VGPRs
 
 thread_local char g_error[512] = "";
 
@@ -87,6 +90,24 @@ __device__ __forceinline__ double gather(double v, int addr)
     return __hiloint2double(hi, lo);
 }
 
+// Value of `v` held by lane ^ 1 (DPP quad_perm [1, 0, 3, 2]).
+__device__ __forceinline__ double swap_pair(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF,
+                                            0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF,
+                                            0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Broadcast lane `src` (wave-uniform) of `v` to the whole wavefront.
+__device__ __forceinline__ double read_lane(double v, int src)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -95,9 +116,9 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Node time of global output interval i, fine division r, node s, evaluated
-// in the reference's order without contraction:
-//   ((t0 + (r dt) / F) + dt i) + (dt c_s) / F
+// Node time of global output interval i, fine division r, node c, in the
+// reference's order without contraction:
+//   ((t0 + (r dt) / F) + dt i) + (dt c) / F
 __device__ double node_time(double t0, double dt, int fine, int r, long i,
                             double c)
 {
@@ -107,15 +128,15 @@ __device__ double node_time(double t0, double dt, int fine, int r, long i,
     return start + (dt * c) / (double)fine;
 }
 
-// Waveform coefficient g at time t; cursor[g] only moves forward because
-// node times increase.
-__device__ double wave_coef(const ssmx_wave &wv, const ssmx_segment *segs,
+// Waveform coefficient g at time t (per lane); `cursor` only moves forward
+// because each lane's node times increase.
+__device__ double wave_coef(const ssmx_wave *wv, const ssmx_segment *segs,
                             int g, double t, int &cursor)
 {
 #pragma clang fp contract(off)
-    double c = wv.base[g];
-    const int begin = wv.seg_begin[g];
-    const int count = wv.seg_count[g];
+    double c = wv->base[g];
+    const int begin = wv->seg_begin[g];
+    const int count = wv->seg_count[g];
     while (cursor < count && !(t < segs[begin + cursor].t_end))
         ++cursor;
     if (cursor < count) {
@@ -126,102 +147,152 @@ __device__ double wave_coef(const ssmx_wave &wv, const ssmx_segment *segs,
     return c;
 }
 
-template <int SLOTS>
+// Scalar factor of matrix m in exponential j at global step `step`:
+//   non-rotating (m = g):      h * sum_s w[j,s] c_g(t_s)
+//   rotating (m = s * K + g):  c_g(t_s) * (h * w[j,s])
+__device__ double step_factor(const Params &p, long step, int j, int m,
+                              int &cursor)
+{
+    const ssmx_propagate_desc &d = p.d;
+    const int S = d.n_node, K = d.n_gen;
+    const long i_loc = step / d.fine;
+    const int r = (int)(step - i_loc * d.fine);
+    const long ig = (long)d.first_out + i_loc;
+    const bool wave_mode = (d.flags & SSMX_COEF_WAVE) != 0;
+    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
+    const double *row = nullptr;
+    if (!wave_mode)
+        row = p.coef + (size_t)0 + ((size_t)(i_loc * d.fine + r) * S) * K;
+    if (rotating) {
+        const int s = m / K, g = m % K;
+        const double c =
+            wave_mode ? wave_coef(p.waves, p.segs, g,
+                                  node_time(d.t0, d.dt, d.fine, r, ig,
+                                            p.nodes[s]),
+                                  cursor)
+                      : row[s * K + g];
+        return c * (d.h * p.w[j * S + s]);
+    }
+    double cw = 0.0;
+    for (int s = 0; s < S; ++s) {
+        const double c =
+            wave_mode ? wave_coef(p.waves, p.segs, m,
+                                  node_time(d.t0, d.dt, d.fine, r, ig,
+                                            p.nodes[s]),
+                                  cursor)
+                      : row[s * K + m];
+        cw = fma(p.w[j * S + s], c, cw);
+    }
+    return d.h * cw;
+}
+
+template <int SLOTS, bool VREG, bool GLDS>
 __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     Params p)
 {
     extern __shared__ double lds_vals[];
     const ssmx_propagate_desc &d = p.d;
     const int lane = threadIdx.x & (kWave - 1);
-    const int nmat = ((d.flags & SSMX_ROTATING) ? d.n_node : 1) * d.n_gen;
-    const int nval = nmat * SLOTS * kWave;
-    for (int e = threadIdx.x; e < nval; e += blockDim.x) {
-        const int slot = (e / kWave) % SLOTS;
-        const int m = e / (kWave * SLOTS);
-        const int row = e % kWave;
-        lds_vals[e] = slot < d.slots
-                          ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
-                                      row]
-                          : 0.0;
+    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
+    const int M = (rotating ? d.n_node : 1) * d.n_gen;  // matrices
+    if (!VREG) {
+        const int nval = M * SLOTS * kWave;
+        for (int e = threadIdx.x; e < nval; e += blockDim.x) {
+            const int slot = (e / kWave) % SLOTS;
+            const int m = e / (kWave * SLOTS);
+            const int row = e % kWave;
+            lds_vals[e] =
+                slot < d.slots
+                    ? p.ell_val[((size_t)m * d.slots + slot) * kWave + row]
+                    : 0.0;
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
     const int inst = __builtin_amdgcn_readfirstlane(
         (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave));
     if (inst >= d.n_inst)
         return;  // whole wavefront; no barrier follows
+    // Per-wavefront exchange buffer (LDS gathers): 64 doubles after the
+    // generator values.
+    double *xbuf = lds_vals + (VREG ? 0 : M * SLOTS * kWave) +
+                   (threadIdx.x / kWave) * kWave;
 
     const int n = d.n;
-    const int K = d.n_gen;
-    const int S = d.n_node;
     const int J = d.n_exp;
-    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
     const bool wave_mode = (d.flags & SSMX_COEF_WAVE) != 0;
     const bool read_full = (d.flags & SSMX_READ_FULL) != 0;
     const bool read_int = (d.flags & SSMX_READ_INTEGRAL) != 0;
+    const bool use_partner = d.partner != 0;
+    const int nslots = d.slots;
 
+    // Per-instance views.
+    Params q = p;
+    if (wave_mode)
+        q.waves = p.waves + inst;
+    else
+        q.coef = p.coef + (size_t)inst * d.coef_inst_stride;
+
+    // Register copy of the generator values (VREG) and gather addresses.
+    double vreg[VREG ? kRegVals : 1];
+    if (VREG) {
+#pragma unroll
+        for (int e = 0; e < kRegVals; ++e) {
+            const int m = e / SLOTS, slot = e % SLOTS;
+            vreg[e] = (m < M && slot < d.slots)
+                          ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
+                                      lane]
+                          : 0.0;
+        }
+    }
     int addr[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; ++s)
-        addr[s] = 4 * (s < d.slots ? p.ell_col[s * kWave + lane] : 0);
+        addr[s] = 4 * (s < d.slots ? p.ell_col[s * kWave + lane] : lane);
 
     double x = lane < n ? p.rho0[(size_t)inst * n + lane] : 0.0;
 
     double eig_own = 0.0, eig_part = 0.0;
-    const int part_addr = 4 * (lane ^ 1);
     if (rotating) {
         eig_own = p.eig[lane];
         eig_part = p.eig[kWave + lane];
     }
-
-    double obs_row[SSMX_MAX_OBS];
-    double obs_off[SSMX_MAX_OBS];
-    const bool read_obs = !read_full;
-#pragma unroll
-    for (int o = 0; o < SSMX_MAX_OBS; ++o) {
-        const bool use = read_obs && o < d.n_obs;
-        obs_row[o] = use ? p.obs[o * kWave + lane] : 0.0;
-        obs_off[o] = (use && p.obs_offset)
-                         ? p.obs_offset[(size_t)inst * d.n_obs + o]
-                         : 0.0;
+    double obs0 = 0.0, obs1 = 0.0, off0 = 0.0, off1 = 0.0;
+    if (!read_full) {
+        obs0 = p.obs[lane];
+        off0 = p.obs_offset ? p.obs_offset[(size_t)inst * d.n_obs] : 0.0;
+        if (d.n_obs > 1) {
+            obs1 = p.obs[kWave + lane];
+            off1 = p.obs_offset ? p.obs_offset[(size_t)inst * d.n_obs + 1]
+                                : 0.0;
+        }
     }
 
-    const ssmx_wave *wv = wave_mode ? p.waves + inst : nullptr;
-    int cursor[SSMX_MAX_GEN];
-#pragma unroll
-    for (int g = 0; g < SSMX_MAX_GEN; ++g)
-        cursor[g] = 0;
-    const double *table =
-        wave_mode ? nullptr : p.coef + (size_t)inst * d.coef_inst_stride;
+    // Lane-parallel coefficient batches: entry e = (b * J + j) * M + m.
+    const int JM = J * M;
+    const int B = kWave / JM;  // steps per batch (host guarantees >= 1)
+    const long total = (long)d.n_out * d.fine;
+    const int my_b = lane / JM;
+    const int my_j = (lane % JM) / M;
+    const int my_m = lane % M;
+    int cursor = 0;
+    double factor = 0.0;
 
     const double inv_sub = 1.0 / (double)d.substeps;
-    double xint = 0.0;  // trapezoid integral of the state
+    double xint = 0.0;
     const double half_h = 0.5 * d.h;
 
+    long step = 0;
+    int b = 0;
     for (int i = 0; i < d.n_out; ++i) {
-        const long ig = (long)d.first_out + i;
-        for (int r = 0; r < d.fine; ++r) {
-            // Coefficients of every node, uniform across the wavefront.
-            double c[SSMX_MAX_NODE][SSMX_MAX_GEN];
-#pragma unroll
-            for (int s = 0; s < SSMX_MAX_NODE; ++s) {
-                if (s >= S)
-                    break;
-                if (wave_mode) {
-                    const double t =
-                        node_time(d.t0, d.dt, d.fine, r, ig, p.nodes[s]);
-#pragma unroll
-                    for (int g = 0; g < SSMX_MAX_GEN; ++g)
-                        c[s][g] = g < K ? wave_coef(*wv, p.segs, g, t,
-                                                    cursor[g])
-                                        : 0.0;
-                } else {
-                    const double *row =
-                        table + ((size_t)(i * d.fine + r) * S + s) * K;
-#pragma unroll
-                    for (int g = 0; g < SSMX_MAX_GEN; ++g)
-                        c[s][g] = g < K ? row[g] : 0.0;
-                }
+        for (int r = 0; r < d.fine; ++r, ++step, ++b) {
+            if (b == B)
+                b = 0;
+            if (b == 0) {
+                const long mine = step + my_b;
+                factor = (my_b < B && mine < total)
+                             ? step_factor(q, mine, my_j, my_m, cursor)
+                             : 0.0;
             }
             const double x_prev = x;
             for (int j = J - 1; j >= 0; --j) {
@@ -229,46 +300,26 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
 #pragma unroll
                 for (int sl = 0; sl < SLOTS; ++sl)
                     a[sl] = 0.0;
-                if (rotating) {
+                const int base = (b * J + j) * M;
+                if (VREG) {
 #pragma unroll
-                    for (int s = 0; s < SSMX_MAX_NODE; ++s) {
-                        if (s >= S)
+                    for (int m = 0; m < kRegVals / SLOTS; ++m) {
+                        if (m >= M)
                             break;
-                        const double hw = d.h * p.w[j * S + s];
-#pragma unroll
-                        for (int g = 0; g < SSMX_MAX_GEN; ++g) {
-                            if (g >= K)
-                                break;
-                            const double *v =
-                                lds_vals + ((s * K + g) * SLOTS) * kWave +
-                                lane;
-                            const double cg = c[s][g];
-#pragma unroll
-                            for (int sl = 0; sl < SLOTS; ++sl)
-                                a[sl] = fma(v[sl * kWave] * cg, hw, a[sl]);
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int g = 0; g < SSMX_MAX_GEN; ++g) {
-                        if (g >= K)
-                            break;
-                        double cw = 0.0;
-#pragma unroll
-                        for (int s = 0; s < SSMX_MAX_NODE; ++s) {
-                            if (s >= S)
-                                break;
-                            cw = fma(p.w[j * S + s], c[s][g], cw);
-                        }
-                        const double *v =
-                            lds_vals + (g * SLOTS) * kWave + lane;
+                        const double f = read_lane(factor, base + m);
 #pragma unroll
                         for (int sl = 0; sl < SLOTS; ++sl)
-                            a[sl] = fma(cw, v[sl * kWave], a[sl]);
+                            a[sl] = fma(f, vreg[m * SLOTS + sl], a[sl]);
                     }
+                } else {
+                    for (int m = 0; m < M; ++m) {
+                        const double f = read_lane(factor, base + m);
+                        const double *v =
+                            lds_vals + (m * SLOTS) * kWave + lane;
 #pragma unroll
-                    for (int sl = 0; sl < SLOTS; ++sl)
-                        a[sl] *= d.h;
+                        for (int sl = 0; sl < SLOTS; ++sl)
+                            a[sl] = fma(f, v[sl * kWave], a[sl]);
+                    }
                 }
                 if (d.substeps > 1) {
 #pragma unroll
@@ -278,19 +329,53 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                 for (int sub = 0; sub < d.substeps; ++sub) {
                     double term = x;
                     double acc = x;
-                    for (int q = 1; q <= d.taylor; ++q) {
-                        double y = 0.0;
+                    for (int t = 1; t <= d.taylor; ++t) {
+                        double g[SLOTS];
+                        g[0] = term;
+                        if (GLDS) {
+                            xbuf[lane] = term;
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE,
+                                                   "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,
+                                                   "wavefront");
+                        }
+                        if (SLOTS > 1)
+                            g[1] = use_partner ? swap_pair(term)
+                                   : GLDS      ? xbuf[addr[1] >> 2]
+                                               : gather(term, addr[1]);
 #pragma unroll
-                        for (int sl = 0; sl < SLOTS; ++sl)
-                            y = fma(a[sl], gather(term, addr[sl]), y);
-                        term = y * kInv[q];
+                        for (int sl = 2; sl < SLOTS; ++sl) {
+                            if (sl < nslots)
+                                g[sl] = GLDS ? xbuf[addr[sl] >> 2]
+                                             : gather(term, addr[sl]);
+                            else
+                                g[sl] = 0.0;
+                        }
+                        if (GLDS) {
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE,
+                                                   "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,
+                                                   "wavefront");
+                        }
+                        double y0 = a[0] * g[0];
+                        double y1 = 0.0;
+#pragma unroll
+                        for (int sl = 1; sl < SLOTS; ++sl) {
+                            if (sl & 1)
+                                y1 = fma(a[sl], g[sl], y1);
+                            else
+                                y0 = fma(a[sl], g[sl], y0);
+                        }
+                        term = (y0 + y1) * kInv[t];
                         acc += term;
                     }
                     x = acc;
                 }
             }
             if (rotating) {
-                const double xp = gather(x, part_addr);
+                const double xp = swap_pair(x);
                 x = x + fma(eig_own, x, eig_part * xp);
             }
             if (read_int)
@@ -300,57 +385,73 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
             if (lane < n)
                 p.out[((size_t)inst * d.n_out + i) * n + lane] = x;
         } else if (!read_int) {
-#pragma unroll
-            for (int o = 0; o < SSMX_MAX_OBS; ++o) {
-                if (o >= d.n_obs)
-                    break;
-                const double v = wave_sum(obs_row[o] * x);
+            const double v0 = wave_sum(obs0 * x);
+            if (lane == 0)
+                p.out[((size_t)inst * d.n_out + i) * d.n_obs] = v0 + off0;
+            if (d.n_obs > 1) {
+                const double v1 = wave_sum(obs1 * x);
                 if (lane == 0)
-                    p.out[((size_t)inst * d.n_out + i) * d.n_obs + o] =
-                        v + obs_off[o];
+                    p.out[((size_t)inst * d.n_out + i) * d.n_obs + 1] =
+                        v1 + off1;
             }
         }
     }
     if (read_int) {
-#pragma unroll
-        for (int o = 0; o < SSMX_MAX_OBS; ++o) {
-            if (o >= d.n_obs)
-                break;
-            const double v = wave_sum(obs_row[o] * xint);
+        const double v0 = wave_sum(obs0 * xint);
+        if (lane == 0)
+            p.out[(size_t)inst * d.n_obs] = v0 + off0;
+        if (d.n_obs > 1) {
+            const double v1 = wave_sum(obs1 * xint);
             if (lane == 0)
-                p.out[(size_t)inst * d.n_obs + o] = v + obs_off[o];
+                p.out[(size_t)inst * d.n_obs + 1] = v1 + off1;
         }
     }
     if (p.state_out && lane < n)
         p.state_out[(size_t)inst * n + lane] = x;
 }
 
-template <int SLOTS>
-int launch(const Params &p, hipStream_t stream)
+template <int SLOTS, bool VREG, bool GLDS>
+int launch_one(const Params &p, hipStream_t stream)
 {
-    const int nmat =
-        ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
-    const size_t lds = (size_t)nmat * SLOTS * kWave * sizeof(double);
+    const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
+    const size_t lds =
+        (VREG ? 0 : (size_t)M * SLOTS * kWave * sizeof(double)) +
+        (GLDS ? (size_t)kWavesPerBlock * kWave * sizeof(double) : 0);
     if (lds > 160 * 1024)
         return set_error(SSMX_EINVAL,
                          "generator values need %zu bytes of LDS (> 160 KiB)",
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&propagate_kernel<SLOTS>),
+            reinterpret_cast<const void *>(
+                &propagate_kernel<SLOTS, VREG, GLDS>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
     const int blocks = (p.d.n_inst + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(propagate_kernel<SLOTS>, dim3(blocks),
+    hipLaunchKernelGGL((propagate_kernel<SLOTS, VREG, GLDS>), dim3(blocks),
                        dim3(kWave * kWavesPerBlock), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return set_error(SSMX_ELAUNCH, "propagate launch: %s",
                          hipGetErrorString(e));
     return SSMX_OK;
+}
+
+template <int SLOTS>
+int launch(const Params &p, hipStream_t stream)
+{
+    const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
+    // Register-resident values only pay off while occupancy stays >= 4
+    // wavefronts per SIMD, i.e. for the smallest patterns.
+    const bool glds = (p.d.flags & SSMX_GATHER_LDS) != 0;
+    if (SLOTS <= 4 && M * SLOTS <= kRegVals)
+        return glds ? launch_one<SLOTS, true, true>(p, stream)
+                    : launch_one<SLOTS, true, false>(p, stream);
+    return glds ? launch_one<SLOTS, false, true>(p, stream)
+                : launch_one<SLOTS, false, false>(p, stream);
 }
 
 int validate(const ssmx_propagate_desc *d)
@@ -372,6 +473,11 @@ int validate(const ssmx_propagate_desc *d)
     if (d->n_node < 1 || d->n_node > SSMX_MAX_NODE)
         return set_error(SSMX_EINVAL, "n_node=%d outside 1..%d", d->n_node,
                          SSMX_MAX_NODE);
+    const int M = ((d->flags & SSMX_ROTATING) ? d->n_node : 1) * d->n_gen;
+    if (d->n_exp * M > kWave)
+        return set_error(SSMX_EINVAL,
+                         "n_exp * matrices = %d exceeds one batch of 64",
+                         d->n_exp * M);
     if (d->fine < 1 || d->n_out < 0)
         return set_error(SSMX_EINVAL, "fine=%d, n_out=%d", d->fine, d->n_out);
     if (d->taylor < 1 || d->taylor > SSMX_MAX_TAYLOR)
@@ -382,10 +488,8 @@ int validate(const ssmx_propagate_desc *d)
         return set_error(SSMX_EINVAL,
                          "substeps=%d is not a power of two <= 1024",
                          d->substeps);
-    if (!(d->flags & SSMX_READ_FULL) &&
-        (d->n_obs < 1 || d->n_obs > SSMX_MAX_OBS))
-        return set_error(SSMX_EINVAL, "n_obs=%d outside 1..%d", d->n_obs,
-                         SSMX_MAX_OBS);
+    if (!(d->flags & SSMX_READ_FULL) && (d->n_obs < 1 || d->n_obs > 2))
+        return set_error(SSMX_EINVAL, "n_obs=%d outside 1..2", d->n_obs);
     if ((d->flags & SSMX_READ_FULL) && (d->flags & SSMX_READ_INTEGRAL))
         return set_error(SSMX_EINVAL,
                          "READ_FULL and READ_INTEGRAL are exclusive");
@@ -393,6 +497,8 @@ int validate(const ssmx_propagate_desc *d)
         return set_error(SSMX_EINVAL, "h and dt must be positive");
     if (d->first_out < 0)
         return set_error(SSMX_EINVAL, "first_out=%d < 0", d->first_out);
+    if (d->partner != 0 && d->partner != 1)
+        return set_error(SSMX_EINVAL, "partner flag must be 0 or 1");
     return SSMX_OK;
 }
 
@@ -400,7 +506,7 @@ int validate(const ssmx_propagate_desc *d)
 
 extern "C" {
 
-int ssmx_version(void) { return 100; }
+int ssmx_version(void) { return 101; }
 
 const char *ssmx_last_error(void) { return g_error; }
 
@@ -457,6 +563,7 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
 
     const int slots = desc->slots;
     if (slots <= 4) return launch<4>(p, s);
+    if (slots <= 6) return launch<6>(p, s);
     if (slots <= 8) return launch<8>(p, s);
     if (slots <= 12) return launch<12>(p, s);
     if (slots <= 16) return launch<16>(p, s);

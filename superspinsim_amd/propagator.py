@@ -42,15 +42,40 @@ def _ptr(t):
 # ELL packing
 # ---------------------------------------------------------------------------
 
+def pair_permutation(pattern):
+    """Basis order that puts coupled index pairs at (2k, 2k + 1).
+
+    ``pattern`` is the (n, n) boolean union of the generators' non-zeros.
+    Pairs (a, b) coupled both ways are matched greedily (most couplings
+    first); matched pairs come first, unmatched indices after.  Returns
+    ``perm`` with ``new[k] = old[perm[k]]``.
+    """
+    n = pattern.shape[0]
+    weight = pattern.astype(int) + pattern.T.astype(int)
+    cand = [(weight[a, b], a, b) for a in range(n) for b in range(a + 1, n)
+            if pattern[a, b] and pattern[b, a]]
+    cand.sort(key=lambda t: (-t[0], t[1], t[2]))
+    used = np.zeros(n, dtype=bool)
+    order = []
+    for _, a, b in cand:
+        if not used[a] and not used[b]:
+            used[a] = used[b] = True
+            order += [a, b]
+    order += [k for k in range(n) if not used[k]]
+    return np.array(order, dtype=np.int64)
+
+
 class EllStack:
     """Generators packed in one ELL pattern (the union of their non-zeros).
 
-    ``mats`` has shape (M, n, n); ``col`` is int32 (slots, 64) and ``val``
-    float64 (M, slots, 64), with rows >= n and padded slots set to
-    column 0 / value 0.
+    ``mats`` has shape (M, n, n).  Slot 0 of row r is always column r (the
+    diagonal); with ``partner`` slot 1 is always column r ^ 1; the other
+    non-zeros of the union fill slots 2.. in column order.  ``col`` is int32
+    (slots, 64), ``val`` float64 (M, slots, 64); unused slots point at
+    column r with value 0, rows >= n are all zero.
     """
 
-    def __init__(self, mats):
+    def __init__(self, mats, partner=True):
         mats = np.asarray(mats, dtype=_f64)
         if mats.ndim != 3 or mats.shape[1] != mats.shape[2]:
             raise ValueError(f"generator stack must be (M, n, n), got "
@@ -59,19 +84,31 @@ class EllStack:
         if n > _lib.MAX_N:
             raise ValueError(f"superoperator dimension {n} > {_lib.MAX_N}")
         union = np.any(mats != 0, axis=0)
-        rows = [np.nonzero(union[r])[0] for r in range(n)]
-        slots = max(1, max(len(c) for c in rows))
-        col = np.zeros((slots, 64), dtype=np.int32)
+        fixed = 2 if partner else 1
+        rows = []
+        for r in range(n):
+            special = {r} | ({r ^ 1} if partner else set())
+            rows.append([c for c in np.nonzero(union[r])[0]
+                         if c not in special])
+        slots = fixed + max(len(c) for c in rows)
+        col = np.tile(np.arange(64, dtype=np.int32), (slots, 1))
         val = np.zeros((M, slots, 64), dtype=_f64)
         for r in range(n):
+            val[:, 0, r] = mats[:, r, r]
+            if partner:
+                col[1, r] = r ^ 1
+                if (r ^ 1) < n:
+                    val[:, 1, r] = mats[:, r, r ^ 1]
             for k, c in enumerate(rows[r]):
-                col[k, r] = c
-                val[:, k, r] = mats[:, r, c]
+                col[fixed + k, r] = c
+                val[:, fixed + k, r] = mats[:, r, c]
         self.n = n
         self.slots = slots
+        self.partner = bool(partner)
         self.col = col
         self.val = val
         self.nnz = int(union.sum())
+        self.gather_slots = slots - fixed
         self.norms = np.abs(mats).sum(axis=1).max(axis=1)  # 1-norms
 
 
@@ -214,7 +251,8 @@ class Propagator:
     eigen-step, or None.
     """
 
-    def __init__(self, mats, weights, nodes, rotating=False, eig=None):
+    def __init__(self, mats, weights, nodes, rotating=False, eig=None,
+                 permute=None):
         torch = _torch()
         _lib.load()
         mats = np.asarray(mats, dtype=_f64)
@@ -239,13 +277,27 @@ class Propagator:
             flat = mats
         if self.K > _lib.MAX_GEN:
             raise ValueError(f"{self.K} generators > {_lib.MAX_GEN}")
-        self.ell = EllStack(flat)
+        # The rotating eigenbasis already holds its 2x2 blocks at
+        # (2k, 2k + 1); other bases are reordered so that coupled pairs are.
+        if permute is None:
+            permute = not self.rotating
+        n0 = flat.shape[-1]
+        if permute:
+            perm = pair_permutation(np.any(flat != 0, axis=0))
+        else:
+            perm = np.arange(n0)
+        self.perm = perm
+        self.inv_perm = np.argsort(perm)
+        flat = flat[:, perm][:, :, perm]
+        self.ell = EllStack(flat, partner=True)
         self.n = self.ell.n
         norms = self.ell.norms
         self.gen_norms = norms.reshape(self.S, self.K) if self.rotating \
             else norms
         self.weights = weights
         self.nodes = nodes
+        self.d_perm = device_tensor(perm)
+        self.d_inv_perm = device_tensor(self.inv_perm)
         self.d_col = device_tensor(self.ell.col)
         self.d_val = device_tensor(self.ell.val)
         self.d_w = device_tensor(weights)
@@ -270,7 +322,7 @@ class Propagator:
     def run(self, *, n_inst, fine, n_out, h, t0, dt, rho0, coef=None,
             coef_inst_stride=0, waves=None, segments=None, taylor=None,
             substeps=1, read="full", obs=None, obs_offset=None, out=None,
-            state_out=None, first_out=0, stream=None):
+            state_out=None, first_out=0, stream=None, gather="lds"):
         """Launch one propagation; returns the output tensor.
 
         ``rho0`` (n_inst, n) device tensor.  Coefficients: ``coef`` device
@@ -297,6 +349,10 @@ class Propagator:
                 raise ValueError(f"unknown readout {read!r}")
         if waves is not None:
             flags |= _lib.SSMX_COEF_WAVE
+        if gather == "lds":
+            flags |= _lib.SSMX_GATHER_LDS
+        elif gather != "bpermute":
+            raise ValueError(f"unknown gather {gather!r}")
         if out is None:
             out = torch.empty(shape, dtype=torch.float64, device="cuda")
         if taylor is None:
@@ -305,14 +361,18 @@ class Propagator:
             n=self.n, n_inst=n_inst, n_gen=self.K, slots=self.ell.slots,
             n_exp=self.J, n_node=self.S, fine=fine, n_out=n_out,
             taylor=int(taylor), substeps=int(substeps), n_obs=max(n_obs, 1),
-            flags=flags, first_out=int(first_out), reserved=0,
+            flags=flags, first_out=int(first_out),
+            partner=1 if self.ell.partner else 0,
             coef_inst_stride=int(coef_inst_stride), h=float(h),
             t0=float(t0), dt=float(dt))
         obs_rows = None
         if obs is not None:
             rows = np.zeros((n_obs, 64), dtype=_f64)
-            rows[:, :self.n] = obs
+            rows[:, :self.n] = np.asarray(obs)[:, self.perm]
             obs_rows = device_tensor(rows)
+        rho0 = rho0[:, self.d_perm].contiguous()
+        if state_out is not None:
+            raise ValueError("use run_state for the final state")
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         lib = _lib.load()
@@ -324,5 +384,7 @@ class Propagator:
             _ptr(out), _ptr(state_out), ctypes.c_void_p(stream))
         _lib.check(rc)
         # keep the readout rows alive until the stream has consumed them
-        self._keep = obs_rows
+        self._keep = (obs_rows, rho0)
+        if read == "full":
+            out = out[..., self.d_inv_perm]
         return out
