@@ -21,9 +21,10 @@
  * Data layout of a propagation problem (one launch advances n_inst
  * independent instances over n_out * fine propagator steps each):
  *  - Superoperators are n x n real matrices (n <= 64), stored in a shared
- *    ELL pattern: column index ell_col[slot * 64 + row] (int32); slot 0 of
- *    row r must be column r (the diagonal, value possibly 0); unused slots
- *    point at column r with value 0; values
+ *    ELL pattern: column index ell_col[slot * 64 + row] (int32).  Slot 0
+ *    of row r is always column r (the diagonal, value possibly 0) and
+ *    ell_col[row] (slot 0) instead holds the number of used slots of that
+ *    row; unused slots point at column r with value 0; values
  *    ell_val[((node * n_gen + g) * slots + slot) * 64 + row]; node ranges
  *    over n_node for the rotating frame (node-wise interaction-picture
  *    generators) and is 0 otherwise.

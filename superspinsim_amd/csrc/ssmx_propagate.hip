@@ -196,11 +196,15 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     const bool rotating = (d.flags & SSMX_ROTATING) != 0;
     const int M = (rotating ? d.n_node : 1) * d.n_gen;  // matrices
     if (!VREG) {
+        // Staged as [m][slot / 2][lane][slot % 2] so that one ds_read_b128
+        // returns two slots of a row.
         const int nval = M * SLOTS * kWave;
         for (int e = threadIdx.x; e < nval; e += blockDim.x) {
-            const int slot = (e / kWave) % SLOTS;
+            const int half = e & 1;
+            const int row = (e >> 1) % kWave;
+            const int pair = (e / (2 * kWave)) % (SLOTS / 2);
             const int m = e / (kWave * SLOTS);
-            const int row = e % kWave;
+            const int slot = 2 * pair + half;
             lds_vals[e] =
                 slot < d.slots
                     ? p.ell_val[((size_t)m * d.slots + slot) * kWave + row]
@@ -224,7 +228,6 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     const bool read_full = (d.flags & SSMX_READ_FULL) != 0;
     const bool read_int = (d.flags & SSMX_READ_INTEGRAL) != 0;
     const bool use_partner = d.partner != 0;
-    const int nslots = d.slots;
 
     // Per-instance views.
     Params q = p;
@@ -245,10 +248,16 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                           : 0.0;
         }
     }
+    // Slot 0 is the diagonal by definition (ell_col[0][row] holds the row's
+    // used-slot count for callers that want it); other slots hold columns.
     int addr[SLOTS];
+    addr[0] = 4 * lane;
 #pragma unroll
-    for (int s = 0; s < SLOTS; ++s)
+    for (int s = 1; s < SLOTS; ++s)
         addr[s] = 4 * (s < d.slots ? p.ell_col[s * kWave + lane] : lane);
+    // Lane t holds 1/t: the Taylor recursion broadcasts it with readlane
+    // instead of a scalar memory load inside the dependent chain.
+    const double inv_lane = kInv[lane <= SSMX_MAX_TAYLOR ? lane : 0];
 
     double x = lane < n ? p.rho0[(size_t)inst * n + lane] : 0.0;
 
@@ -314,11 +323,15 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                 } else {
                     for (int m = 0; m < M; ++m) {
                         const double f = read_lane(factor, base + m);
-                        const double *v =
-                            lds_vals + (m * SLOTS) * kWave + lane;
+                        const double2 *v = reinterpret_cast<const double2 *>(
+                                               lds_vals + m * SLOTS * kWave) +
+                                           lane;
 #pragma unroll
-                        for (int sl = 0; sl < SLOTS; ++sl)
-                            a[sl] = fma(f, v[sl * kWave], a[sl]);
+                        for (int pr = 0; pr < SLOTS / 2; ++pr) {
+                            const double2 vv = v[pr * kWave];
+                            a[2 * pr] = fma(f, vv.x, a[2 * pr]);
+                            a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
+                        }
                     }
                 }
                 if (d.substeps > 1) {
@@ -344,14 +357,12 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                             g[1] = use_partner ? swap_pair(term)
                                    : GLDS      ? xbuf[addr[1] >> 2]
                                                : gather(term, addr[1]);
+                        // Unused slots point at the row itself with value
+                        // 0, so every slot is read without predication.
 #pragma unroll
-                        for (int sl = 2; sl < SLOTS; ++sl) {
-                            if (sl < nslots)
-                                g[sl] = GLDS ? xbuf[addr[sl] >> 2]
-                                             : gather(term, addr[sl]);
-                            else
-                                g[sl] = 0.0;
-                        }
+                        for (int sl = 2; sl < SLOTS; ++sl)
+                            g[sl] = GLDS ? xbuf[addr[sl] >> 2]
+                                         : gather(term, addr[sl]);
                         if (GLDS) {
                             __builtin_amdgcn_fence(__ATOMIC_RELEASE,
                                                    "wavefront");
@@ -368,7 +379,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                             else
                                 y0 = fma(a[sl], g[sl], y0);
                         }
-                        term = (y0 + y1) * kInv[t];
+                        term = (y0 + y1) * read_lane(inv_lane, t);
                         acc += term;
                     }
                     x = acc;

@@ -91,6 +91,7 @@ class EllStack:
             rows.append([c for c in np.nonzero(union[r])[0]
                          if c not in special])
         slots = fixed + max(len(c) for c in rows)
+        slots += slots % 2      # the kernel reads slot pairs
         col = np.tile(np.arange(64, dtype=np.int32), (slots, 1))
         val = np.zeros((M, slots, 64), dtype=_f64)
         for r in range(n):
@@ -102,6 +103,11 @@ class EllStack:
             for k, c in enumerate(rows[r]):
                 col[fixed + k, r] = c
                 val[:, fixed + k, r] = mats[:, r, c]
+        # slot 0 is the diagonal by definition; its column entry carries the
+        # number of used slots of each row
+        col[0, :] = fixed
+        for r in range(n):
+            col[0, r] = fixed + len(rows[r])
         self.n = n
         self.slots = slots
         self.partner = bool(partner)
@@ -125,6 +131,29 @@ def taylor_remainder(x, m):
     return x ** (m + 1) / math.factorial(m + 1) / (1 - x / (m + 2))
 
 
+def coefficient_samples(cmin, cmax, count=48, seed=0):
+    """Node-coefficient samples (count, S, K) covering the box [cmin, cmax].
+
+    Corners of the varying coordinates (up to 2^8), the centre and uniform
+    random points; constant coordinates (cmin == cmax) are fixed.
+    """
+    cmin = np.asarray(cmin, dtype=_f64)
+    cmax = np.asarray(cmax, dtype=_f64)
+    vary = np.nonzero((cmax - cmin).ravel() != 0)[0]
+    rng = np.random.default_rng(seed)
+    out = [0.5 * (cmin + cmax)]
+    if vary.size <= 8:
+        for bits in range(2 ** vary.size):
+            c = cmin.copy().ravel()
+            for k, v in enumerate(vary):
+                if (bits >> k) & 1:
+                    c[v] = cmax.ravel()[v]
+            out.append(c.reshape(cmin.shape))
+    for _ in range(count):
+        out.append(cmin + (cmax - cmin) * rng.random(cmin.shape))
+    return np.array(out)
+
+
 def taylor_plan(norm, tol=1e-17, max_taylor=_lib.MAX_TAYLOR):
     """Cheapest (m, s) with s * remainder(norm / s, m) <= tol.
 
@@ -145,6 +174,47 @@ def taylor_plan(norm, tol=1e-17, max_taylor=_lib.MAX_TAYLOR):
     if best is None:
         raise ValueError(f"no Taylor plan reaches tol={tol} for "
                          f"norm={norm}")
+    return best
+
+
+def taylor_plan_measured(exponents, tol=1e-17, safety=8.0,
+                         max_taylor=_lib.MAX_TAYLOR):
+    """(m, s) from exact powers of representative exponents A.
+
+    For each candidate substep count s and degree m the truncation error of
+    (T_m(A/s))^s x is bounded by
+        s * ||B^(m+1)||_1 / (m+1)! / (1 - ||B||_1 / (m+2)),  B = A/s,
+    which is rigorous for the sampled A (||B^(m+1+i)|| <= ||B^(m+1)||
+    ||B||^i); ``safety`` covers coefficients between the samples.  Returns
+    the cheapest (m * s) plan meeting ``tol``.
+    """
+    exps = [np.asarray(a, dtype=_f64) for a in exponents]
+    best = None
+    s = 1
+    while s <= 1024:
+        scaled = [a / s for a in exps]
+        norms = [float(np.abs(b).sum(axis=0).max()) for b in scaled]
+        powers = [b.copy() for b in scaled]
+        fact = 1.0
+        for m in range(1, max_taylor + 1):
+            fact *= (m + 1)
+            worst = 0.0
+            for k, b in enumerate(scaled):
+                powers[k] = powers[k] @ b          # B^(m+1)
+                if norms[k] >= m + 2:
+                    worst = math.inf
+                    break
+                pn = float(np.abs(powers[k]).sum(axis=0).max())
+                worst = max(worst, pn / fact / (1 - norms[k] / (m + 2)))
+            if s * safety * worst <= tol:
+                if best is None or m * s < best[0] * best[1]:
+                    best = (m, s)
+                break
+        s *= 2
+        if best is not None and best[0] * best[1] <= s:
+            break
+    if best is None:
+        raise ValueError(f"no Taylor plan reaches tol={tol}")
     return best
 
 
@@ -296,6 +366,7 @@ class Propagator:
             else norms
         self.weights = weights
         self.nodes = nodes
+        self._dense = mats.copy()
         self.d_perm = device_tensor(perm)
         self.d_inv_perm = device_tensor(self.inv_perm)
         self.d_col = device_tensor(self.ell.col)
@@ -304,6 +375,25 @@ class Propagator:
         self.d_nodes = device_tensor(nodes)
         self.d_eig = device_tensor(eig) if eig is not None else None
         self.torch = torch
+
+    def sample_exponents(self, coef_samples, h):
+        """Dense A_j for node-coefficient samples (count, S, K), all j."""
+        out = []
+        mats = self._dense
+        for c in np.asarray(coef_samples, dtype=_f64):
+            for j in range(self.J):
+                if self.rotating:
+                    f = c * (h * self.weights[j][:, None])     # (S, K)
+                    out.append(np.tensordot(f, mats, axes=([0, 1], [0, 1])))
+                else:
+                    cw = self.weights[j] @ c                   # (K,)
+                    out.append(h * np.tensordot(cw, mats, axes=(0, 0)))
+        return out
+
+    def plan(self, coef_samples, h, tol=1e-17):
+        """Taylor plan from sampled exponents (see taylor_plan_measured)."""
+        return taylor_plan_measured(self.sample_exponents(coef_samples, h),
+                                    tol=tol)
 
     def norm_bound(self, coef_abs_max, h):
         """Bound on ||A_j||_1 from per-node maxima |c[s, g]| (S, K)."""

@@ -22,8 +22,8 @@ import numpy as np
 
 from superspinsim_amd import _lib, quadratures
 from superspinsim_amd.propagator import (
-    Propagator, device_tensor, eig_lane_factors, rotating_tables,
-    taylor_plan)
+    Propagator, coefficient_samples, device_tensor, eig_lane_factors,
+    rotating_tables)
 
 _f64 = np.float64
 
@@ -59,6 +59,17 @@ class Waveform:
                 if ts <= t < te:
                     out[g] += amp * math.cos(om * t + ph)
         return out
+
+    def bounds(self):
+        """(lower, upper) bounds of c_g(t) over all t."""
+        lo = self.base.copy()
+        hi = self.base.copy()
+        for g, segs in enumerate(self.segments):
+            if segs:
+                a = max(abs(s[2]) for s in segs)
+                lo[g] -= a
+                hi[g] += a
+        return lo, hi
 
     def abs_max(self):
         """Upper bound of |c_g(t)| over all t."""
@@ -118,7 +129,7 @@ class BatchSimulator:
     def __init__(self, generators, number_of_exponentials=2,
                  number_of_fine_divisions=1, use_rotating=False,
                  vectors_real=None, inv_vectors_real=None, doubles=None,
-                 singles=None, tolerance=1e-17):
+                 singles=None, tolerance=1e-16):
         self.gens = np.asarray(generators, dtype=_f64)
         self.G, self.n = self.gens.shape[0], self.gens.shape[1]
         self.nodes, self.weights, self.label = quadratures.scheme(
@@ -157,8 +168,12 @@ class BatchSimulator:
         active = [g for g in range(self.G) if amax[g] != 0] or [0]
         prop = self._propagator(active, time_step)
         h = time_step / self.F
-        cmax = np.broadcast_to(amax[active], (self.nodes.size, len(active)))
-        m, s = taylor_plan(prop.norm_bound(cmax, h), tol=self.tolerance)
+        lo = np.min([w.bounds()[0] for w in waves], axis=0)[active]
+        hi = np.max([w.bounds()[1] for w in waves], axis=0)[active]
+        S = self.nodes.size
+        samples = coefficient_samples(np.broadcast_to(lo, (S, lo.size)),
+                                      np.broadcast_to(hi, (S, hi.size)))
+        m, s = prop.plan(samples, h, tol=self.tolerance)
         return prop, active, m, s
 
     def run(self, waves, rho0, time_start, time_end, time_step, obs=None,
