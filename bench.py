@@ -35,8 +35,6 @@ def parse():
     p.add_argument("--cpu-outputs", type=int, default=None,
                    help="output intervals of the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--gather", default="lds", choices=["lds", "bpermute"],
-                   help="neighbour exchange of the sparse products")
     p.add_argument("--outputs", type=int, default=None,
                    help="truncate every instance to this many output "
                         "intervals (profiling runs)")
@@ -123,15 +121,10 @@ def main():
     plan = dict(sim.last_plan)
 
     # Pre-pack so the timed region is launch + kernel only.
-    prop, active, m, s = sim.plan(waves, wl.time_step)
-    from superspinsim_amd.sweep import Waveform, pack_waveforms
-    sub = []
-    for w in waves:
-        nw = Waveform(len(active), w.base[active])
-        for k, g in enumerate(active):
-            nw.segments[k] = list(w.segments[g])
-        sub.append(nw)
-    d_waves, d_segs = pack_waveforms(sub, len(active))
+    prop, program, m, s = sim.plan(waves, wl.time_step)
+    from superspinsim_amd.sweep import pack_waveforms
+    d_waves, d_segs = pack_waveforms(sim.kernel_waves(waves, program),
+                                     len(program))
     from superspinsim_amd.propagator import device_tensor as dt_
     d_rho = dt_(np.repeat(wl.rho0[None], n_inst, axis=0))
     out = torch.empty((n_inst, wl.n_out, 1), dtype=torch.float64,
@@ -142,7 +135,7 @@ def main():
                  h=wl.time_step / wl.fine, t0=wl.time_start,
                  dt=wl.time_step, rho0=d_rho, waves=d_waves,
                  segments=d_segs, taylor=m, substeps=s, read="obs",
-                 obs=wl.obs, out=out, gather=args.gather)
+                 obs=wl.obs, out=out)
 
     launch()
     torch.cuda.synchronize()
@@ -197,7 +190,6 @@ def main():
             "ell_slots": plan["slots"],
             "nnz": plan["nnz"],
             "parallelism": "one instance per wavefront",
-            "gather": args.gather,
         },
         "roofline": {
             "bound": "mfma",

@@ -58,9 +58,6 @@ extern "C" {
 #define SSMX_COEF_WAVE 2     /* coefficients from waveform descriptors */
 #define SSMX_READ_FULL 4     /* readout: whole state vector (n values) */
 #define SSMX_READ_INTEGRAL 8 /* readout: trapezoid integral of obs rows */
-#define SSMX_GATHER_LDS 16   /* exchange neighbour components through an
-                                LDS copy of the state (ds_read_b64) instead
-                                of ds_bpermute */
 
 /* limits */
 #define SSMX_MAX_N 64

@@ -10,13 +10,12 @@ import ctypes
 import os
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libssmx.so")
+LIB_PATH = os.environ.get("SSMX_LIB") or os.path.join(LIB_DIR, "libssmx.so")
 
 SSMX_ROTATING = 1
 SSMX_COEF_WAVE = 2
 SSMX_READ_FULL = 4
 SSMX_READ_INTEGRAL = 8
-SSMX_GATHER_LDS = 16
 
 MAX_N = 64
 MAX_GEN = 8

@@ -21,8 +21,11 @@
 //  - slot 0 of every ELL row is the diagonal: no exchange;
 //  - slot 1 is the partner column r ^ 1 (the Re/Im pair of a coherence, or
 //    a 2x2 eigen-block): exchanged with a DPP quad permutation (VALU);
-//  - slots 2.. are arbitrary columns: fetched with ds_bpermute (LDS
-//    crossbar, no bank conflicts).
+//  - slots 2.. are arbitrary columns: each lane writes its component to a
+//    per-wavefront LDS buffer once, then reads its columns with ds_read_b64
+//    (LDS instructions of one wavefront execute in order, so no barrier is
+//    needed; half as many LDS instructions as two 32-bit ds_bpermute per
+//    double).
 // Coefficients are evaluated lane-parallel in batches: lane e of a batch
 // computes the scalar factor of (step, exponential j, matrix m); the factors
 // are broadcast to the wavefront with readlane when the step runs.
@@ -42,7 +45,6 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
-constexpr int kRegVals = 24;  // max matrix values per lane kept in VGPRs
 
 thread_local char g_error[512] = "";
 
@@ -81,14 +83,6 @@ struct Params {
     double *out;
     double *state_out;
 };
-
-// Value of `v` held by the lane whose byte address (4 * lane) is `addr`.
-__device__ __forceinline__ double gather(double v, int addr)
-{
-    const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
-    const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
-    return __hiloint2double(hi, lo);
-}
 
 // Value of `v` held by lane ^ 1 (DPP quad_perm [1, 0, 3, 2]).
 __device__ __forceinline__ double swap_pair(double v)
@@ -186,29 +180,31 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
     return d.h * cw;
 }
 
-template <int SLOTS, bool VREG, bool GLDS>
+template <int SLOTS>
 __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     Params p)
 {
-    extern __shared__ double lds_vals[];
+    // LDS: generator values staged as [m][slot / 2][lane][slot % 2] (one
+    // ds_read_b128 returns two slots of a row), then one 64-double exchange
+    // buffer per wavefront.
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    extern __shared__ double lds[];
     const ssmx_propagate_desc &d = p.d;
     const int lane = threadIdx.x & (kWave - 1);
     const bool rotating = (d.flags & SSMX_ROTATING) != 0;
     const int M = (rotating ? d.n_node : 1) * d.n_gen;  // matrices
-    if (!VREG) {
-        // Staged as [m][slot / 2][lane][slot % 2] so that one ds_read_b128
-        // returns two slots of a row.
-        const int nval = M * SLOTS * kWave;
+    {
+        const int nval = M * SLOTP * kWave;
         for (int e = threadIdx.x; e < nval; e += blockDim.x) {
             const int half = e & 1;
             const int row = (e >> 1) % kWave;
-            const int pair = (e / (2 * kWave)) % (SLOTS / 2);
-            const int m = e / (kWave * SLOTS);
+            const int pair = (e / (2 * kWave)) % (SLOTP / 2);
+            const int m = e / (kWave * SLOTP);
             const int slot = 2 * pair + half;
-            lds_vals[e] =
-                slot < d.slots
-                    ? p.ell_val[((size_t)m * d.slots + slot) * kWave + row]
-                    : 0.0;
+            lds[e] = slot < d.slots
+                         ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
+                                     row]
+                         : 0.0;
         }
         __syncthreads();
     }
@@ -217,10 +213,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
         (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave));
     if (inst >= d.n_inst)
         return;  // whole wavefront; no barrier follows
-    // Per-wavefront exchange buffer (LDS gathers): 64 doubles after the
-    // generator values.
-    double *xbuf = lds_vals + (VREG ? 0 : M * SLOTS * kWave) +
-                   (threadIdx.x / kWave) * kWave;
+    double *xbuf = lds + M * SLOTP * kWave + (threadIdx.x / kWave) * kWave;
 
     const int n = d.n;
     const int J = d.n_exp;
@@ -236,27 +229,18 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     else
         q.coef = p.coef + (size_t)inst * d.coef_inst_stride;
 
-    // Register copy of the generator values (VREG) and gather addresses.
-    double vreg[VREG ? kRegVals : 1];
-    if (VREG) {
-#pragma unroll
-        for (int e = 0; e < kRegVals; ++e) {
-            const int m = e / SLOTS, slot = e % SLOTS;
-            vreg[e] = (m < M && slot < d.slots)
-                          ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
-                                      lane]
-                          : 0.0;
-        }
-    }
     // Slot 0 is the diagonal by definition (ell_col[0][row] holds the row's
-    // used-slot count for callers that want it); other slots hold columns.
-    int addr[SLOTS];
-    addr[0] = 4 * lane;
+    // used-slot count); slot 1 is the partner r ^ 1 when `partner` is set;
+    // other slots hold column indices.  Unused slots point at the row
+    // itself with value 0, so every slot is read without predication.
+    int col[SLOTS];
+    col[0] = lane;
 #pragma unroll
     for (int s = 1; s < SLOTS; ++s)
-        addr[s] = 4 * (s < d.slots ? p.ell_col[s * kWave + lane] : lane);
+        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
     // Lane t holds 1/t: the Taylor recursion broadcasts it with readlane
-    // instead of a scalar memory load inside the dependent chain.
+    // (a scalar memory load or a division would sit in the dependent
+    // chain of every term).
     const double inv_lane = kInv[lane <= SSMX_MAX_TAYLOR ? lane : 0];
 
     double x = lane < n ? p.rho0[(size_t)inst * n + lane] : 0.0;
@@ -279,7 +263,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
 
     // Lane-parallel coefficient batches: entry e = (b * J + j) * M + m.
     const int JM = J * M;
-    const int B = kWave / JM;  // steps per batch (host guarantees >= 1)
+    const int B = kWave / JM;  // steps per batch (validated >= 1)
     const long total = (long)d.n_out * d.fine;
     const int my_b = lane / JM;
     const int my_j = (lane % JM) / M;
@@ -290,6 +274,34 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     const double inv_sub = 1.0 / (double)d.substeps;
     double xint = 0.0;
     const double half_h = 0.5 * d.h;
+
+    // y = A v for the current ELL row values a[] (one Taylor product).
+    auto product = [&](const double (&a)[SLOTS], double v) -> double {
+        double g[SLOTS];
+        g[0] = v;
+        xbuf[lane] = v;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (SLOTS > 1)
+            g[1] = use_partner ? swap_pair(v) : xbuf[col[1]];
+#pragma unroll
+        for (int sl = 2; sl < SLOTS; ++sl)
+            g[sl] = xbuf[col[sl]];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double y0 = a[0] * g[0];
+        double y1 = 0.0;
+#pragma unroll
+        for (int sl = 1; sl < SLOTS; ++sl) {
+            if (sl & 1)
+                y1 = fma(a[sl], g[sl], y1);
+            else
+                y0 = fma(a[sl], g[sl], y0);
+        }
+        return y0 + y1;
+    };
 
     long step = 0;
     int b = 0;
@@ -310,28 +322,18 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                 for (int sl = 0; sl < SLOTS; ++sl)
                     a[sl] = 0.0;
                 const int base = (b * J + j) * M;
-                if (VREG) {
+                for (int m = 0; m < M; ++m) {
+                    const double f = read_lane(factor, base + m);
+                    const double2 *v =
+                        reinterpret_cast<const double2 *>(lds +
+                                                          m * SLOTP * kWave) +
+                        lane;
 #pragma unroll
-                    for (int m = 0; m < kRegVals / SLOTS; ++m) {
-                        if (m >= M)
-                            break;
-                        const double f = read_lane(factor, base + m);
-#pragma unroll
-                        for (int sl = 0; sl < SLOTS; ++sl)
-                            a[sl] = fma(f, vreg[m * SLOTS + sl], a[sl]);
-                    }
-                } else {
-                    for (int m = 0; m < M; ++m) {
-                        const double f = read_lane(factor, base + m);
-                        const double2 *v = reinterpret_cast<const double2 *>(
-                                               lds_vals + m * SLOTS * kWave) +
-                                           lane;
-#pragma unroll
-                        for (int pr = 0; pr < SLOTS / 2; ++pr) {
-                            const double2 vv = v[pr * kWave];
-                            a[2 * pr] = fma(f, vv.x, a[2 * pr]);
+                    for (int pr = 0; pr < SLOTP / 2; ++pr) {
+                        const double2 vv = v[pr * kWave];
+                        a[2 * pr] = fma(f, vv.x, a[2 * pr]);
+                        if (2 * pr + 1 < SLOTS)
                             a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
-                        }
                     }
                 }
                 if (d.substeps > 1) {
@@ -343,43 +345,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                     double term = x;
                     double acc = x;
                     for (int t = 1; t <= d.taylor; ++t) {
-                        double g[SLOTS];
-                        g[0] = term;
-                        if (GLDS) {
-                            xbuf[lane] = term;
-                            __builtin_amdgcn_fence(__ATOMIC_RELEASE,
-                                                   "wavefront");
-                            __builtin_amdgcn_wave_barrier();
-                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,
-                                                   "wavefront");
-                        }
-                        if (SLOTS > 1)
-                            g[1] = use_partner ? swap_pair(term)
-                                   : GLDS      ? xbuf[addr[1] >> 2]
-                                               : gather(term, addr[1]);
-                        // Unused slots point at the row itself with value
-                        // 0, so every slot is read without predication.
-#pragma unroll
-                        for (int sl = 2; sl < SLOTS; ++sl)
-                            g[sl] = GLDS ? xbuf[addr[sl] >> 2]
-                                         : gather(term, addr[sl]);
-                        if (GLDS) {
-                            __builtin_amdgcn_fence(__ATOMIC_RELEASE,
-                                                   "wavefront");
-                            __builtin_amdgcn_wave_barrier();
-                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE,
-                                                   "wavefront");
-                        }
-                        double y0 = a[0] * g[0];
-                        double y1 = 0.0;
-#pragma unroll
-                        for (int sl = 1; sl < SLOTS; ++sl) {
-                            if (sl & 1)
-                                y1 = fma(a[sl], g[sl], y1);
-                            else
-                                y0 = fma(a[sl], g[sl], y0);
-                        }
-                        term = (y0 + y1) * read_lane(inv_lane, t);
+                        term = product(a, term) * read_lane(inv_lane, t);
                         acc += term;
                     }
                     x = acc;
@@ -421,48 +387,34 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
         p.state_out[(size_t)inst * n + lane] = x;
 }
 
-template <int SLOTS, bool VREG, bool GLDS>
-int launch_one(const Params &p, hipStream_t stream)
+template <int SLOTS>
+int launch(const Params &p, hipStream_t stream)
 {
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
     const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
-    const size_t lds =
-        (VREG ? 0 : (size_t)M * SLOTS * kWave * sizeof(double)) +
-        (GLDS ? (size_t)kWavesPerBlock * kWave * sizeof(double) : 0);
+    const size_t lds = ((size_t)M * SLOTP * kWave +
+                        (size_t)kWavesPerBlock * kWave) *
+                       sizeof(double);
     if (lds > 160 * 1024)
         return set_error(SSMX_EINVAL,
                          "generator values need %zu bytes of LDS (> 160 KiB)",
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(
-                &propagate_kernel<SLOTS, VREG, GLDS>),
+            reinterpret_cast<const void *>(&propagate_kernel<SLOTS>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
     const int blocks = (p.d.n_inst + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((propagate_kernel<SLOTS, VREG, GLDS>), dim3(blocks),
+    hipLaunchKernelGGL((propagate_kernel<SLOTS>), dim3(blocks),
                        dim3(kWave * kWavesPerBlock), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return set_error(SSMX_ELAUNCH, "propagate launch: %s",
                          hipGetErrorString(e));
     return SSMX_OK;
-}
-
-template <int SLOTS>
-int launch(const Params &p, hipStream_t stream)
-{
-    const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
-    // Register-resident values only pay off while occupancy stays >= 4
-    // wavefronts per SIMD, i.e. for the smallest patterns.
-    const bool glds = (p.d.flags & SSMX_GATHER_LDS) != 0;
-    if (SLOTS <= 4 && M * SLOTS <= kRegVals)
-        return glds ? launch_one<SLOTS, true, true>(p, stream)
-                    : launch_one<SLOTS, true, false>(p, stream);
-    return glds ? launch_one<SLOTS, false, true>(p, stream)
-                : launch_one<SLOTS, false, false>(p, stream);
 }
 
 int validate(const ssmx_propagate_desc *d)
@@ -517,7 +469,7 @@ int validate(const ssmx_propagate_desc *d)
 
 extern "C" {
 
-int ssmx_version(void) { return 101; }
+int ssmx_version(void) { return 102; }
 
 const char *ssmx_last_error(void) { return g_error; }
 
@@ -572,15 +524,25 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
     p.state_out = state_out;
     hipStream_t s = static_cast<hipStream_t>(stream);
 
-    const int slots = desc->slots;
-    if (slots <= 4) return launch<4>(p, s);
-    if (slots <= 6) return launch<6>(p, s);
-    if (slots <= 8) return launch<8>(p, s);
-    if (slots <= 12) return launch<12>(p, s);
-    if (slots <= 16) return launch<16>(p, s);
-    if (slots <= 24) return launch<24>(p, s);
-    if (slots <= 32) return launch<32>(p, s);
-    if (slots <= 48) return launch<48>(p, s);
+    switch (desc->slots) {
+    case 1: case 2: return launch<2>(p, s);
+    case 3: return launch<3>(p, s);
+    case 4: return launch<4>(p, s);
+    case 5: return launch<5>(p, s);
+    case 6: return launch<6>(p, s);
+    case 7: return launch<7>(p, s);
+    case 8: return launch<8>(p, s);
+    case 9: return launch<9>(p, s);
+    case 10: return launch<10>(p, s);
+    case 11: case 12: return launch<12>(p, s);
+    case 13: case 14: return launch<14>(p, s);
+    case 15: case 16: return launch<16>(p, s);
+    default: break;
+    }
+    if (desc->slots <= 20) return launch<20>(p, s);
+    if (desc->slots <= 24) return launch<24>(p, s);
+    if (desc->slots <= 32) return launch<32>(p, s);
+    if (desc->slots <= 48) return launch<48>(p, s);
     return launch<64>(p, s);
 }
 

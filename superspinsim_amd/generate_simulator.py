@@ -162,19 +162,37 @@ def generate_simulator(
 
         coef = tabulate(sampler, gens.shape[0], N, F, nodes, time_start,
                         time_step)
-        active = np.nonzero(np.any(coef != 0, axis=(0, 1, 2)))[0]
-        if active.size == 0:
-            active = np.array([0])
-        coef = np.ascontiguousarray(coef[..., active])
-
+        # Generators whose coefficient never changes are summed into one
+        # matrix with coefficient 1; all-zero coefficients are dropped.
+        cols = coef.reshape(-1, coef.shape[-1])
+        active = [g for g in range(cols.shape[1]) if np.any(cols[:, g] != 0)]
+        const = {g: float(cols[0, g]) for g in active
+                 if np.all(cols[:, g] == cols[0, g])}
+        varying = [g for g in active if g not in const]
+        if not active:
+            const = {0: 0.0}
         if use_rotating:
-            mats, d_step, s_step = rotating_tables(
+            gen_all, d_step, s_step = rotating_tables(
                 gens, doubles, singles, nodes, time_step, F, use_residual)
-            mats = mats[:, active]
             eig = eig_lane_factors(d_step, s_step, n, use_residual)
+            parts = []
+            if const:
+                parts.append(sum(v * gen_all[:, g]
+                                 for g, v in const.items())[:, None])
+            if varying:
+                parts.append(gen_all[:, varying])
+            mats = np.concatenate(parts, axis=1)
         else:
-            mats = gens[active]
+            parts = []
+            if const:
+                parts.append(sum(v * gens[g] for g, v in const.items())[None])
+            if varying:
+                parts.append(gens[varying])
+            mats = np.concatenate(parts, axis=0)
             eig = None
+        ones = [np.ones(coef.shape[:-1] + (1,))] if const else []
+        coef = np.ascontiguousarray(np.concatenate(
+            ones + [coef[..., varying]], axis=-1))
         prop = Propagator(mats, weights, nodes, rotating=use_rotating,
                           eig=eig)
         # exponents of actual steps: the 48 with the largest coefficient

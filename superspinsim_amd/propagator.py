@@ -91,7 +91,6 @@ class EllStack:
             rows.append([c for c in np.nonzero(union[r])[0]
                          if c not in special])
         slots = fixed + max(len(c) for c in rows)
-        slots += slots % 2      # the kernel reads slot pairs
         col = np.tile(np.arange(64, dtype=np.int32), (slots, 1))
         val = np.zeros((M, slots, 64), dtype=_f64)
         for r in range(n):
@@ -412,7 +411,7 @@ class Propagator:
     def run(self, *, n_inst, fine, n_out, h, t0, dt, rho0, coef=None,
             coef_inst_stride=0, waves=None, segments=None, taylor=None,
             substeps=1, read="full", obs=None, obs_offset=None, out=None,
-            state_out=None, first_out=0, stream=None, gather="lds"):
+            state_out=None, first_out=0, stream=None):
         """Launch one propagation; returns the output tensor.
 
         ``rho0`` (n_inst, n) device tensor.  Coefficients: ``coef`` device
@@ -439,10 +438,6 @@ class Propagator:
                 raise ValueError(f"unknown readout {read!r}")
         if waves is not None:
             flags |= _lib.SSMX_COEF_WAVE
-        if gather == "lds":
-            flags |= _lib.SSMX_GATHER_LDS
-        elif gather != "bpermute":
-            raise ValueError(f"unknown gather {gather!r}")
         if out is None:
             out = torch.empty(shape, dtype=torch.float64, device="cuda")
         if taylor is None:

@@ -145,17 +145,36 @@ class BatchSimulator:
             self.singles = np.asarray(singles, dtype=_f64)
         self._prop_cache = {}
 
-    def _propagator(self, active, time_step):
-        key = (tuple(active), float(time_step))
+    def _propagator(self, active, const, time_step):
+        """Propagator over [folded constants] + varying generators.
+
+        ``const`` maps generator -> its time-independent coefficient (same
+        for every instance); those generators are summed into one matrix
+        with coefficient 1, which the kernel assembles like any other.
+        """
+        key = (tuple(active), tuple(sorted(const.items())), float(time_step))
         if key not in self._prop_cache:
+            varying = [g for g in active if g not in const]
             if self.rotating:
                 mats, d_step, s_step = rotating_tables(
                     self.gens, self.doubles, self.singles, self.nodes,
                     time_step, self.F, True)
-                mats = mats[:, active]
                 eig = eig_lane_factors(d_step, s_step, self.n, True)
+                parts = []
+                if const:
+                    parts.append(sum(v * mats[:, g] for g, v in
+                                     const.items())[:, None])
+                if varying:
+                    parts.append(mats[:, varying])
+                mats = np.concatenate(parts, axis=1)
             else:
-                mats = self.gens[active]
+                parts = []
+                if const:
+                    parts.append(sum(v * self.gens[g] for g, v in
+                                     const.items())[None])
+                if varying:
+                    parts.append(self.gens[varying])
+                mats = np.concatenate(parts, axis=0)
                 eig = None
             self._prop_cache[key] = Propagator(
                 mats, self.weights, self.nodes, rotating=self.rotating,
@@ -163,18 +182,47 @@ class BatchSimulator:
         return self._prop_cache[key]
 
     def plan(self, waves, time_step):
-        """(propagator, active generators, Taylor degree, substeps)."""
+        """(propagator, coefficient program, Taylor degree, substeps).
+
+        The coefficient program is a list of (generator index or None,
+        ...) describing the kernel's generators: None is the folded sum of
+        constant generators (coefficient 1), an int a varying generator.
+        """
         amax = np.max([w.abs_max() for w in waves], axis=0)
         active = [g for g in range(self.G) if amax[g] != 0] or [0]
-        prop = self._propagator(active, time_step)
+        const = {}
+        for g in active:
+            bases = {float(w.base[g]) for w in waves}
+            if len(bases) == 1 and not any(w.segments[g] for w in waves):
+                const[g] = bases.pop()
+        prop = self._propagator(active, const, time_step)
+        program = ([None] if const else []) + \
+            [g for g in active if g not in const]
         h = time_step / self.F
-        lo = np.min([w.bounds()[0] for w in waves], axis=0)[active]
-        hi = np.max([w.bounds()[1] for w in waves], axis=0)[active]
+        lo = np.array([1.0 if g is None else
+                       np.min([w.bounds()[0][g] for w in waves])
+                       for g in program])
+        hi = np.array([1.0 if g is None else
+                       np.max([w.bounds()[1][g] for w in waves])
+                       for g in program])
         S = self.nodes.size
         samples = coefficient_samples(np.broadcast_to(lo, (S, lo.size)),
                                       np.broadcast_to(hi, (S, hi.size)))
         m, s = prop.plan(samples, h, tol=self.tolerance)
-        return prop, active, m, s
+        return prop, program, m, s
+
+    @staticmethod
+    def kernel_waves(waves, program):
+        """Per-instance waveforms over the kernel's generator list."""
+        out = []
+        for w in waves:
+            base = [1.0 if g is None else w.base[g] for g in program]
+            nw = Waveform(len(program), base)
+            for k, g in enumerate(program):
+                if g is not None:
+                    nw.segments[k] = list(w.segments[g])
+            out.append(nw)
+        return out
 
     def run(self, waves, rho0, time_start, time_end, time_step, obs=None,
             read="obs", taylor=None, substeps=None):
@@ -187,18 +235,13 @@ class BatchSimulator:
         """
         torch = __import__("torch")
         N = int((time_end - time_start) / time_step)
-        prop, active, m, s = self.plan(waves, time_step)
+        prop, program, m, s = self.plan(waves, time_step)
         if taylor is not None:
             m = taylor
         if substeps is not None:
             s = substeps
-        sub = []
-        for w in waves:
-            nw = Waveform(len(active), w.base[active])
-            for k, g in enumerate(active):
-                nw.segments[k] = list(w.segments[g])
-            sub.append(nw)
-        d_waves, d_segs = pack_waveforms(sub, len(active))
+        d_waves, d_segs = pack_waveforms(self.kernel_waves(waves, program),
+                                         len(program))
         rho0 = np.atleast_2d(np.asarray(rho0, dtype=_f64))
         if rho0.shape[0] == 1 and len(waves) > 1:
             rho0 = np.repeat(rho0, len(waves), axis=0)
@@ -215,7 +258,7 @@ class BatchSimulator:
         if read == "full" and self.rotating:
             out = out @ torch.from_numpy(self.V.T.copy()).to(out.device)
         self.last_plan = dict(taylor=m, substeps=s, slots=prop.ell.slots,
-                              nnz=prop.ell.nnz, active=active, n_out=N,
+                              nnz=prop.ell.nnz, active=program, n_out=N,
                               steps=N * self.F, n=self.n)
         self._keep = (d_waves, d_segs)
         return out
