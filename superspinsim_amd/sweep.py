@@ -87,11 +87,14 @@ class Waveform:
         return sample
 
 
-def pack_waveforms(waves, n_gen):
-    """Device buffers (ssmx_wave array, ssmx_segment array) for instances."""
-    recs = np.zeros(len(waves), dtype=np.dtype([
-        ("base", _f64, _lib.MAX_GEN), ("seg_begin", np.int32, _lib.MAX_GEN),
-        ("seg_count", np.int32, _lib.MAX_GEN)]))
+WAVE_DTYPE = np.dtype([("base", _f64, _lib.MAX_GEN),
+                       ("seg_begin", np.int32, _lib.MAX_GEN),
+                       ("seg_count", np.int32, _lib.MAX_GEN)])
+
+
+def waveform_records(waves, n_gen):
+    """Host arrays matching ``ssmx_wave[]`` and ``ssmx_segment[]``."""
+    recs = np.zeros(len(waves), dtype=WAVE_DTYPE)
     segs = []
     for i, w in enumerate(waves):
         if w.n_gen != n_gen:
@@ -108,8 +111,12 @@ def pack_waveforms(waves, n_gen):
             segs.extend(ordered)
     if not segs:
         segs = [(0.0, 0.0, 0.0, 0.0, 0.0)]
-    seg_arr = np.array(segs, dtype=_f64).reshape(-1, 5)
-    assert recs.itemsize == 8 * _lib.MAX_GEN + 8 * _lib.MAX_GEN
+    return recs, np.array(segs, dtype=_f64).reshape(-1, 5)
+
+
+def pack_waveforms(waves, n_gen):
+    """Device buffers (ssmx_wave array, ssmx_segment array) for instances."""
+    recs, seg_arr = waveform_records(waves, n_gen)
     torch = __import__("torch")
     rec_bytes = np.frombuffer(recs.tobytes(), dtype=np.uint8).copy()
     d_waves = torch.from_numpy(rec_bytes).to("cuda")

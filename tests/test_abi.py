@@ -1,0 +1,110 @@
+"""The C ABI: symbols, struct layout and error handling (no GPU needed).
+
+Every entry point declared in include/*.h must be exported by the built
+library; the ctypes mirrors must match the C layout (checked against a
+probe compiled with gcc from the header); invalid descriptors must be
+rejected with SSMX_EINVAL before any HIP call is made.
+"""
+
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+from superspinsim_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = []
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(ssmx_\w+)\s*\(",
+                            text, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_lib.LIB_PATH):
+        from superspinsim_amd.build import build_library
+        build_library()
+    return _lib.load()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    declared = _declared()
+    assert "ssmx_propagate" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(_lib.EXPORTS)
+
+
+def test_struct_layout_matches_header(tmp_path):
+    probe = tmp_path / "probe.c"
+    probe.write_text(r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "ssmx.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(ssmx_propagate_desc),
+         offsetof(ssmx_propagate_desc, coef_inst_stride),
+         offsetof(ssmx_propagate_desc, h), offsetof(ssmx_propagate_desc, dt),
+         sizeof(ssmx_segment), sizeof(ssmx_wave),
+         offsetof(ssmx_wave, seg_count));
+  return 0;
+}
+''')
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o",
+                    str(exe), str(probe)], check=True)
+    got = [int(v) for v in subprocess.run(
+        [str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    D = _lib.PropagateDesc
+    mine = [ctypes.sizeof(D), D.coef_inst_stride.offset, D.h.offset,
+            D.dt.offset, ctypes.sizeof(_lib.Segment),
+            ctypes.sizeof(_lib.Wave), _lib.Wave.seg_count.offset]
+    assert got == mine
+
+
+def _desc(**kw):
+    base = dict(n=4, n_inst=1, n_gen=1, slots=2, n_exp=1, n_node=1, fine=1,
+                n_out=1, taylor=4, substeps=1, n_obs=1,
+                flags=_lib.SSMX_READ_FULL, first_out=0, partner=1,
+                coef_inst_stride=0, h=1e-11, t0=0.0, dt=1e-11)
+    base.update(kw)
+    return _lib.PropagateDesc(**base)
+
+
+@pytest.mark.parametrize("bad", [
+    dict(n=0), dict(n=65), dict(n_inst=0), dict(n_gen=9), dict(slots=0),
+    dict(n_exp=7), dict(n_node=4), dict(taylor=0), dict(taylor=41),
+    dict(substeps=3), dict(h=0.0), dict(first_out=-1), dict(partner=2),
+    dict(flags=0, n_obs=3),
+    dict(flags=_lib.SSMX_READ_FULL | _lib.SSMX_READ_INTEGRAL),
+    dict(n_exp=6, n_node=3, n_gen=8, flags=_lib.SSMX_ROTATING |
+         _lib.SSMX_READ_FULL),
+])
+def test_invalid_descriptor_is_rejected(lib, bad):
+    d = _desc(**bad)
+    dummy = ctypes.c_void_p(8)
+    rc = lib.ssmx_propagate(ctypes.byref(d), *([dummy] * 13))
+    assert rc == -1
+    assert lib.ssmx_last_error().decode()
+
+
+def test_null_pointers_are_rejected(lib):
+    d = _desc()
+    rc = lib.ssmx_propagate(ctypes.byref(d), *([None] * 13))
+    assert rc == -1
+    assert b"null" in lib.ssmx_last_error()
+    rc = lib.ssmx_propagate(None, *([None] * 13))
+    assert rc == -1
+
+
+def test_version(lib):
+    assert lib.ssmx_version() >= 100

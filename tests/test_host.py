@@ -1,0 +1,170 @@
+"""Host-side logic of the HIP path (no device needed).
+
+ELL packing, basis permutation, Taylor planning, rotating tables, node
+times, flattening and waveform records must be exact, because the kernel
+consumes them verbatim.
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from oracle.reference_simulate import (
+    flatten, node_coefficients, rotating_precompute, unflatten)
+from superspinsim_amd import configs, quadratures
+from superspinsim_amd.generate_generators import make_rotating
+from superspinsim_amd.generate_simulator import _flatten, _unflatten, tabulate
+from superspinsim_amd.propagator import (
+    EllStack, coefficient_samples, eig_lane_factors, pair_permutation,
+    rotating_tables, taylor_plan, taylor_plan_measured, taylor_remainder)
+from superspinsim_amd.sweep import WAVE_DTYPE, Waveform, waveform_records
+
+
+def _dense_from_ell(ell):
+    M = ell.val.shape[0]
+    out = np.zeros((M, ell.n, ell.n))
+    for sl in range(ell.slots):
+        for r in range(ell.n):
+            c = r if sl == 0 else ell.col[sl, r]
+            if c < ell.n:
+                out[:, r, c] += ell.val[:, sl, r]
+    return out
+
+
+def test_ell_roundtrip_and_slot_conventions():
+    wl = configs.odmr7(1)
+    mats = wl.generators[[0, 1, 4]]
+    perm = pair_permutation(np.any(mats != 0, axis=0))
+    assert sorted(perm) == list(range(mats.shape[1]))
+    pm = mats[:, perm][:, :, perm]
+    ell = EllStack(pm)
+    assert np.array_equal(_dense_from_ell(ell), pm)
+    assert np.all(ell.col[1, :ell.n] == np.arange(ell.n) ^ 1)
+    # slot 0 carries the used-slot count of every row
+    assert np.all(ell.col[0] >= 2) and ell.col[0].max() == ell.slots
+
+
+def test_pair_permutation_pairs_coherences():
+    """On the NV basis every Re/Im pair of a coherence becomes (2k, 2k+1)."""
+    wl = configs.odmr7(1)
+    mats = wl.generators[[0, 1, 4]]
+    perm = pair_permutation(np.any(mats != 0, axis=0))
+    vm = wl.vmap[perm]
+    paired = sum(1 for k in range(0, len(perm) - 1, 2)
+                 if tuple(vm[k][:2]) == tuple(vm[k + 1][:2])
+                 and vm[k][2] != vm[k + 1][2])
+    assert paired == 21  # all off-diagonal (y, x) of the 7-level model
+
+
+@pytest.mark.parametrize("norm", [0.01, 0.19, 0.8, 3.0, 40.0])
+def test_taylor_plan_meets_tolerance(norm):
+    m, s = taylor_plan(norm, tol=1e-16)
+    assert s * taylor_remainder(norm / s, m) <= 1e-16
+    if m > 1:
+        assert s * taylor_remainder(norm / s, m - 1) > 1e-16 or s > 1
+
+
+def test_measured_plan_bounds_actual_truncation():
+    rng = np.random.default_rng(3)
+    A = 0.3 * rng.standard_normal((12, 12)) / math.sqrt(12)
+    m, s = taylor_plan_measured([A], tol=1e-14)
+    x = rng.standard_normal(12)
+    y = x.copy()
+    B = A / s
+    for _ in range(s):
+        term = y.copy()
+        acc = y.copy()
+        for k in range(1, m + 1):
+            term = B @ term / k
+            acc += term
+        y = acc
+    exact = expm(A) @ x
+    assert np.max(np.abs(y - exact)) / np.max(np.abs(exact)) < 1e-13
+    m2, _ = taylor_plan_measured([A], tol=1e-8)
+    assert m2 < m
+
+
+def test_coefficient_samples_cover_corners():
+    lo = np.array([[1.0, -2.0, 0.5]])
+    hi = np.array([[1.0, 2.0, 0.5]])
+    smp = coefficient_samples(lo, hi, count=4)
+    assert smp.shape[1:] == (1, 3)
+    assert np.any(smp[:, 0, 1] == -2.0) and np.any(smp[:, 0, 1] == 2.0)
+    assert np.all(smp[:, 0, 0] == 1.0) and np.all(smp[:, 0, 2] == 0.5)
+
+
+def test_rotating_tables_match_oracle_precompute():
+    wl = configs.odmr7(1, use_incoherent=True)
+    g = wl.generators
+    rot, V, Vi, doubles, singles = make_rotating(g[0], list(g[1:]))
+    rot = np.array(rot)
+    nodes, _, _ = quadratures.scheme(2)
+    for residual in (True, False):
+        a, da, sa = rotating_tables(rot, doubles, singles, nodes, 2e-9, 100,
+                                    residual)
+        b, db, sb = rotating_precompute(rot, doubles, singles, nodes, 2e-9,
+                                        100, residual)
+        assert np.max(np.abs(a - b)) <= 1e-15 * np.abs(b).max()
+        assert np.array_equal(da, db) and np.array_equal(sa, sb)
+
+
+def test_eig_lane_factors_layout():
+    d = np.array([[0.1, 0.2], [0.3, 0.4]])
+    s = np.array([0.5])
+    f = eig_lane_factors(d, s, 5)
+    own, part = f[:64], f[64:]
+    assert own[:5].tolist() == [0.1, 0.1, 0.3, 0.3, 0.5]
+    assert part[:5].tolist() == [-0.2, 0.2, -0.4, 0.4, 0.0]
+
+
+def test_node_times_match_oracle_bit_for_bit():
+    nodes, _, _ = quadratures.scheme(5)
+    rows = []
+    sampler = lambda t, row: row.__setitem__(0, t)  # noqa: E731
+    mine = tabulate(sampler, 1, 7, 13, nodes, 1.5e-7, 3e-9)[..., 0]
+    ref = node_coefficients(sampler, 1, 7, nodes.size, 13, nodes, 1.5e-7,
+                            3e-9)[..., 0]
+    assert np.array_equal(mine, np.transpose(ref, (1, 0, 2)))
+    del rows
+
+
+def test_flatten_roundtrip_matches_oracle():
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+    r = r + r.conj().T
+    vmap = np.array([[y, x, c] for y in range(4) for x in range(y, 4)
+                     for c in ((0,) if x == y else (0, 1))])
+    order = np.argsort([(0 if y == x else 1, y, x, c) for y, x, c in vmap],
+                       axis=0)[:, 0]
+    vmap = vmap[order]
+    for herm, n_d in ((True, vmap.shape[0]), (False, 32)):
+        a = _flatten(r, vmap, herm, n_d)
+        b = flatten(r, vmap, herm, n_d)
+        assert np.array_equal(a, b)
+        back = _unflatten(a[None], vmap, herm, 4)[0]
+        assert np.allclose(back, r)
+        assert np.array_equal(back, unflatten(b[None], vmap, herm, 4)[0])
+
+
+def test_waveform_records_layout():
+    w = Waveform(3, base=[1.0, 0.0, 0.5])
+    w.tone(1, 2e-4, 2.87e9, t_start=0.0, t_end=1e-6)
+    w.add(1, 2e-6, 3e-6, 1e-4)
+    recs, segs = waveform_records([w, w], 3)
+    assert recs.dtype == WAVE_DTYPE and recs.itemsize == 128
+    assert recs["seg_count"][0].tolist()[:3] == [0, 2, 0]
+    assert recs["seg_begin"][1, 1] == 2
+    assert segs.shape == (4, 5)
+    assert segs[0, 3] == math.tau * 2.87e9
+    assert w(0.5e-6)[1] == 2e-4 * math.cos(math.tau * 2.87e9 * 0.5e-6)
+    assert w(1.5e-6)[1] == 0.0 and w(2.5e-6)[1] == 1e-4
+
+
+def test_overlapping_segments_rejected():
+    w = Waveform(1)
+    w.add(0, 0.0, 2.0, 1.0)
+    w.add(0, 1.0, 3.0, 1.0)
+    with pytest.raises(ValueError):
+        waveform_records([w], 1)
