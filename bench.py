@@ -22,6 +22,13 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the default sweep (4096 points, full horizon),
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB copy
+# (fetch x2.0, write x1.0); the summary is kept with the profiles.
+MEASURED_TRAFFIC = {
+    ("odmr7-nv-n49", 4096, 99900): (1334.0 * 2.0 + 32117.0 * 1.0) * 1024,
+}
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -35,6 +42,9 @@ def parse():
     p.add_argument("--cpu-outputs", type=int, default=None,
                    help="output intervals of the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a 1 GiB device-to-device copy (HBM "
+                        "counter calibration for FETCH_SIZE / WRITE_SIZE)")
     p.add_argument("--outputs", type=int, default=None,
                    help="truncate every instance to this many output "
                         "intervals (profiling runs)")
@@ -98,6 +108,16 @@ def main():
     import numpy as np
     import torch
     from superspinsim_amd.propagator import device_tensor  # noqa: F401
+
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+        return
 
     wl = workload(args.config, args.instances)
     if args.outputs:
@@ -197,7 +217,8 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(
+                (wl.name, n_inst, wl.steps_per_instance())),
             "kernel": "propagate_kernel",
             "kernel_ms": avg_kernel_s * 1e3,
             "flops_per_step": flops_step,
