@@ -1,10 +1,17 @@
-"""Benchmark: propagator steps/s of the 7-level NV ODMR sweep, FP64.
+"""Benchmark: propagator steps/s of the 7-level NV ODMR sweep, FP64, and
+the adjoint/forward cost ratio of a full parameter gradient.
 
 One bench step is one whole sweep: every instance (sweep point) advanced
 through all of its propagator steps (h = 20 ps, CF4:2) in one launch of the
 HIP propagator, with the fluorescence read out every output interval.
 Inputs (generators, waveforms, initial states) are resident in HBM before
 the timed region.
+
+The gradient leg runs the 1024-point version of the sweep with a
+200-segment microwave amplitude: forward with checkpoints every 1000 steps,
+then the adjoint (recomputation + reverse sweep) for the squared error
+against targets simulated at parameters perturbed by up to 2 %; the ratio
+is (forward with checkpoints + backward) / plain forward, wall time.
 
 Prints one JSON line (rank 0, N = 1 GPU).
 """
@@ -48,6 +55,12 @@ def parse():
     p.add_argument("--outputs", type=int, default=None,
                    help="truncate every instance to this many output "
                         "intervals (profiling runs)")
+    p.add_argument("--no-adjoint", action="store_true",
+                   help="skip the gradient (adjoint/forward ratio) leg")
+    p.add_argument("--adjoint-only", action="store_true",
+                   help="run only the gradient leg (profiling)")
+    p.add_argument("--adjoint-instances", type=int, default=1024)
+    p.add_argument("--adjoint-steps", type=int, default=2)
     return p.parse_args()
 
 
@@ -99,6 +112,81 @@ def cpu_baseline(wl, outputs):
                       "product (oracle/cpu_ref.c)"}
 
 
+def adjoint_leg(args):
+    """Adjoint/forward cost ratio on the gradient workload (wall time)."""
+    import numpy as np
+    import torch
+    from superspinsim_amd import configs
+    wl = configs.odmr7_gradient(count=args.adjoint_instances)
+    pb = wl.problem
+    t_setup = time.perf_counter()
+    state = pb.setup(wl.theta, wl.amplitudes)
+    target, _ = pb.forward(wl.theta_target, wl.amplitudes_target,
+                           pb.setup(wl.theta_target, wl.amplitudes_target,
+                                    derivatives=False))
+    plain_state = pb.setup(wl.theta, wl.amplitudes, derivatives=False)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    print(f"gradient workload {wl.name}: {state['n_inst']} instances x "
+          f"{pb.n_out * pb.F} steps, setup {setup_s:.1f} s", file=sys.stderr,
+          flush=True)
+
+    def plain():
+        return pb.forward(wl.theta, wl.amplitudes, plain_state)[0]
+
+    def grad():
+        out, st = pb.forward(wl.theta, wl.amplitudes, state)
+        seed = 2.0 * (out - target)
+        res = pb.backward(st, seed)
+        return out, res
+
+    plain()
+    grad()
+    torch.cuda.synchronize()
+    times = {"forward": [], "forward_ckpt": [], "backward": []}
+    for _ in range(args.adjoint_steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out, st = pb.forward(wl.theta, wl.amplitudes, state)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res = pb.backward(st, 2.0 * (out - target))
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        times["forward"].append(t1 - t0)
+        times["forward_ckpt"].append(t2 - t1)
+        times["backward"].append(t3 - t2)
+    med = {k: float(np.median(v)) for k, v in times.items()}
+    g_theta = res["theta"].sum(dim=0).cpu().numpy()
+    g_amp = res["amplitudes"].cpu().numpy()
+    loss = float(((out - target) ** 2).sum())
+    steps = state["n_inst"] * pb.n_out * pb.F
+    return {
+        "workload": wl.name,
+        "instances": state["n_inst"],
+        "steps_per_instance": pb.n_out * pb.F,
+        "amplitude_segments": int(wl.amplitudes.size),
+        "parameters": list(configs.THETA_NAMES),
+        "checkpoint_every": pb.K,
+        "taylor_degree": int(state["taylor"]),
+        "adjoint_degree": int(state["taylor_adj"]),
+        "ell_slots": int(state["prop"].ell.slots),
+        "ell_slots_adjoint": int(state["slots_t"]),
+        "forward_s": med["forward"],
+        "forward_ckpt_s": med["forward_ckpt"],
+        "backward_s": med["backward"],
+        "ratio": (med["forward_ckpt"] + med["backward"]) / med["forward"],
+        "forward_steps_per_s": steps / med["forward"],
+        "loss": loss,
+        "grad_finite": bool(np.all(np.isfinite(g_theta))
+                            and np.all(np.isfinite(g_amp))),
+        "grad_theta": [float(v) for v in g_theta],
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -117,6 +205,10 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
+        return
+
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(args)}))
         return
 
     wl = workload(args.config, args.instances)
@@ -183,7 +275,8 @@ def main():
     ok = bool(np.all(np.isfinite(fluor)))
 
     result = {
-        "metric": "propagator steps/s, 7-level NV ODMR FP64"
+        "metric": "propagator steps/s, 7-level NV ODMR FP64; "
+                  "adjoint/forward cost ratio"
         if args.config.startswith("odmr7") else
         "propagator steps/s, 3-level NV ODMR FP64",
         "value": value,
@@ -225,6 +318,9 @@ def main():
         },
         "finite": ok,
     }
+    if not args.no_adjoint and args.config == "odmr7":
+        result["adjoint"] = adjoint_leg(args)
+        result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]
     if not args.no_cpu:
         outputs = args.cpu_outputs or (128 if wl.n > 20 else 999)
         result["cpu_baseline"] = cpu_baseline(wl, outputs)

@@ -84,6 +84,9 @@ typedef struct ssmx_propagate_desc {
                           launch (node times count intervals from t0) */
     int32_t partner;  /* 1: ELL slot 1 of row r is column r ^ 1 for every
                          row (exchanged by DPP instead of ds_bpermute) */
+    int32_t ckpt_every; /* K > 0: state_out receives checkpoints (the state
+                           before steps 0, K, 2K, ...); 0: the final state */
+    int32_t reserved;   /* 0 */
     int64_t coef_inst_stride; /* table mode: doubles between instances;
                                  0 = one table shared by all instances */
     double h;         /* propagator step dt / F (s) */
@@ -127,7 +130,9 @@ typedef struct ssmx_wave {
  * out        READ_FULL: double [n_inst * n_out * n]
  *            obs: double [n_inst * n_out * n_obs]
  *            READ_INTEGRAL: double [n_inst * n_obs]
- * state_out  double [n_inst * n] final state (or NULL)
+ * state_out  ckpt_every = 0: double [n_inst * n] final state (or NULL);
+ *            ckpt_every = K: double [n_inst * ceil(n_out * fine / K) * n],
+ *            the state before every K-th step (input of ssmx_adjoint)
  *
  * Replaces: generate_simulator.py:1945-2116 (simulate's kernel sequence).
  */
@@ -138,6 +143,98 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
                    const double *eig, const double *obs,
                    const double *obs_offset, double *out, double *state_out,
                    void *stream);
+
+/*
+ * Adjoint (reverse-mode gradient) of the trapezoid-integrated readout
+ *   F_o = sum_k w_k obs_o . x_k,  w_0 = w_T = h / 2, other w_k = h,
+ * over the same propagation as ssmx_propagate with SSMX_READ_INTEGRAL
+ * (non-rotating frame, substeps = 1).  Given seeds g[inst][o] = dLoss/dF_o
+ * it returns, per instance,
+ *   grad_theta[inst][t] = sum_{k,j} f_{kj,theta_gen} <Abar_kj, D_t>
+ *   grad_seg[rec]      += sum_{k,j,s} h w[j,s] dc(t_s)/d amp_rec
+ *                                     <Abar_kj, G_grad_gen>
+ *   lambda_out[inst]    = dLoss/d rho0 (the adjoint state at step 0)
+ * where Abar_kj is the gradient of the loss with respect to the exponent
+ * A_kj of exponential j of step k, f_{kj,m} is the scalar factor of matrix
+ * m in A_kj, and D_t = dG_theta_gen / d theta_t are derivative tables.
+ * Abar is the adjoint of the Taylor action of the block-triangular
+ * augmented exponential exp([[A, E], [0, A]]) (its upper block is the
+ * Frechet derivative of exp at A in direction E), truncated at degree
+ * taylor_adj: Abar = sum_{i+j<taylor_adj} c_ij u_i w_j^T with
+ * u_i = (A^T)^i mu / i!, w_j = A^j y / j!, c_ij = i! j! / (i+j+1)!.
+ *
+ * The forward states are recomputed from checkpoints every ckpt_every steps
+ * (state_out of a ssmx_propagate call with the same ckpt_every); one
+ * wavefront recomputes segment p - 1 while a second one runs the adjoint of
+ * segment p, exchanging states through `workspace`.
+ *
+ * No counterpart in the reference (it has no gradients); it differentiates
+ * the path ssmx_propagate replaces (generate_simulator.py:1945-2116).
+ */
+#define SSMX_ADJ_MAX_TAYLOR 10 /* limit of taylor_adj */
+
+typedef struct ssmx_adjoint_desc {
+    int32_t n;          /* as ssmx_propagate_desc */
+    int32_t n_inst;
+    int32_t n_gen;      /* matrices M in both ELL stacks, 1..8 */
+    int32_t slots;      /* forward ELL slots (pattern of the M matrices) */
+    int32_t slots_t;    /* transposed ELL slots: pattern of the transposed
+                           matrices and derivative tables */
+    int32_t n_exp;
+    int32_t n_node;
+    int32_t fine;
+    int32_t n_out;
+    int32_t taylor;     /* degree of the forward (recomputation) */
+    int32_t taylor_adj; /* degree of the adjoint action, 1..10 */
+    int32_t n_obs;      /* 1..2 */
+    int32_t flags;      /* 0 or SSMX_COEF_WAVE */
+    int32_t first_out;
+    int32_t partner;    /* slot 1 is column r ^ 1 in both ELL stacks */
+    int32_t ckpt_every; /* K, as the forward call that wrote ckpt */
+    int32_t n_theta;    /* derivative tables, 0..16 */
+    int32_t theta_gen;  /* matrix the derivative tables belong to (or -1) */
+    int32_t grad_gen;   /* matrix whose waveform segments get amplitude
+                           gradients (or -1; needs SSMX_COEF_WAVE) */
+    int32_t reserved;   /* 0 */
+    int64_t coef_inst_stride;
+    double h;
+    double t0;
+    double dt;
+} ssmx_adjoint_desc;
+
+#define SSMX_ADJ_MAX_THETA 16
+
+/* Bytes of device workspace ssmx_adjoint needs: n_inst * 2 * ckpt_every *
+ * n_exp * 64 doubles. */
+int ssmx_adjoint_workspace_bytes(const ssmx_adjoint_desc *desc,
+                                 uint64_t *bytes);
+
+/*
+ * ell_col, ell_val   forward ELL of the M matrices (ssmx_propagate layout)
+ * ell_col_t          int32 [slots_t * 64]: ELL of the transposed union
+ *                    pattern; entry (slot, c) names row r of the original
+ *                    matrices (slot 0 = diagonal, holds the used count)
+ * ell_val_t          double [M * slots_t * 64]: G_m[r][c] at (slot, c)
+ * dval_t             double [n_theta * slots_t * 64]: D_t[r][c] at
+ *                    (slot, c) (NULL when n_theta = 0)
+ * cf_weights, cf_nodes, coef, segments   as ssmx_propagate
+ * ckpt               double [n_inst * ceil(n_out * fine / K) * n]
+ * obs                double [n_obs * 64] readout rows
+ * seed               double [n_inst * n_obs] dLoss/dF
+ * workspace          ssmx_adjoint_workspace_bytes() bytes
+ * grad_theta         double [n_inst * n_theta] (or NULL)
+ * grad_seg           double [number of segment records], accumulated
+ *                    with += (zero it first; or NULL)
+ * lambda_out         double [n_inst * n] (or NULL)
+ */
+int ssmx_adjoint(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
+                 const double *ell_val, const int32_t *ell_col_t,
+                 const double *ell_val_t, const double *dval_t,
+                 const double *cf_weights, const double *cf_nodes,
+                 const void *coef, const ssmx_segment *segments,
+                 const double *ckpt, const double *obs, const double *seed,
+                 double *workspace, double *grad_theta, double *grad_seg,
+                 double *lambda_out, void *stream);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *ssmx_last_error(void);

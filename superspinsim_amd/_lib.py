@@ -23,6 +23,9 @@ MAX_EXP = 6
 MAX_NODE = 3
 MAX_OBS = 2
 MAX_TAYLOR = 40
+ADJ_MAX_TAYLOR = 10
+ADJ_MAX_THETA = 16
+ADJ_MAX_SLOTS = 32
 
 
 class PropagateDesc(ctypes.Structure):
@@ -35,6 +38,26 @@ class PropagateDesc(ctypes.Structure):
         ("taylor", ctypes.c_int32), ("substeps", ctypes.c_int32),
         ("n_obs", ctypes.c_int32), ("flags", ctypes.c_int32),
         ("first_out", ctypes.c_int32), ("partner", ctypes.c_int32),
+        ("ckpt_every", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("coef_inst_stride", ctypes.c_int64),
+        ("h", ctypes.c_double), ("t0", ctypes.c_double),
+        ("dt", ctypes.c_double),
+    ]
+
+
+class AdjointDesc(ctypes.Structure):
+    """Mirror of ``ssmx_adjoint_desc`` (include/ssmx.h)."""
+    _fields_ = [
+        ("n", ctypes.c_int32), ("n_inst", ctypes.c_int32),
+        ("n_gen", ctypes.c_int32), ("slots", ctypes.c_int32),
+        ("slots_t", ctypes.c_int32), ("n_exp", ctypes.c_int32),
+        ("n_node", ctypes.c_int32), ("fine", ctypes.c_int32),
+        ("n_out", ctypes.c_int32), ("taylor", ctypes.c_int32),
+        ("taylor_adj", ctypes.c_int32), ("n_obs", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("first_out", ctypes.c_int32),
+        ("partner", ctypes.c_int32), ("ckpt_every", ctypes.c_int32),
+        ("n_theta", ctypes.c_int32), ("theta_gen", ctypes.c_int32),
+        ("grad_gen", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("coef_inst_stride", ctypes.c_int64),
         ("h", ctypes.c_double), ("t0", ctypes.c_double),
         ("dt", ctypes.c_double),
@@ -56,8 +79,8 @@ class Wave(ctypes.Structure):
 
 
 #: every symbol include/ssmx.h declares
-EXPORTS = ("ssmx_propagate", "ssmx_last_error", "ssmx_version",
-           "ssmx_device_count")
+EXPORTS = ("ssmx_propagate", "ssmx_adjoint", "ssmx_adjoint_workspace_bytes",
+           "ssmx_last_error", "ssmx_version", "ssmx_device_count")
 
 _lib = None
 
@@ -79,6 +102,11 @@ def load():
     vp = ctypes.c_void_p
     lib.ssmx_propagate.argtypes = [ctypes.POINTER(PropagateDesc)] + [vp] * 13
     lib.ssmx_propagate.restype = ctypes.c_int
+    lib.ssmx_adjoint.argtypes = [ctypes.POINTER(AdjointDesc)] + [vp] * 17
+    lib.ssmx_adjoint.restype = ctypes.c_int
+    lib.ssmx_adjoint_workspace_bytes.argtypes = [
+        ctypes.POINTER(AdjointDesc), ctypes.POINTER(ctypes.c_uint64)]
+    lib.ssmx_adjoint_workspace_bytes.restype = ctypes.c_int
     lib.ssmx_last_error.argtypes = []
     lib.ssmx_last_error.restype = ctypes.c_char_p
     lib.ssmx_version.argtypes = []

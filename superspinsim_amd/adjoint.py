@@ -1,0 +1,340 @@
+"""Gradients of integrated readouts through the HIP adjoint kernel.
+
+A ``GradientProblem`` is a batch of instances of one parametrised model:
+
+* ``generators(theta)`` returns the (G, n, n) generator stack in the order
+  of the waveform coefficients (for the NV models ``[L0, Gx, Gy, Gz, Gr]``,
+  generate_generators.py:2296-2304);
+* ``waveforms(amplitudes)`` returns one ``Waveform`` per instance; segments
+  whose ``amp`` depends on ``amplitudes[k]`` are added with ``param=k`` and
+  ``dparam = d amp / d amplitudes[k]``;
+* ``rho0``, ``obs`` and the time grid are those of ``BatchSimulator.run``.
+
+The readout is the trapezoid integral over every propagator step,
+``F[i, o] = sum_k w_k obs_o . x_ik`` (``read="integral"``), e.g. the
+integrated fluorescence of an ODMR or pulsed protocol.  ``forward`` runs
+``ssmx_propagate`` and keeps state checkpoints every ``checkpoint_every``
+steps in HBM; ``backward`` runs ``ssmx_adjoint``, which recomputes each
+segment from its checkpoint and returns, per instance, the gradient with
+respect to the physical parameters and to every tagged segment amplitude.
+
+Parameters may enter the generators whose coefficients are the same
+constant for every instance (these are folded into one matrix) and the
+tagged segment amplitudes; ``d generators / d theta`` is evaluated by
+central differences of ``generators`` with step
+``fd_rel * max(|theta_k|, theta_scale_k)``.  ``IntegratedReadout`` wraps
+forward and backward as a ``torch.autograd.Function``.
+
+The reference has no gradients; this differentiates the path that
+``ssmx_propagate`` replaces (generate_simulator.py:1945-2116).
+"""
+
+import ctypes
+
+import numpy as np
+
+from superspinsim_amd import _lib
+from superspinsim_amd.propagator import EllStack, _ptr, _torch, device_tensor
+from superspinsim_amd.sweep import BatchSimulator, waveform_records
+
+_f64 = np.float64
+
+
+class GradientProblem:
+    """Batch of parametrised instances with an integrated readout."""
+
+    def __init__(self, generators, waveforms, rho0, obs, time_start,
+                 time_end, time_step, number_of_fine_divisions=1,
+                 number_of_exponentials=2, checkpoint_every=1000,
+                 tolerance=1e-16, adjoint_tolerance=1e-13, fd_rel=1e-6,
+                 theta_scale=None):
+        self.generators = generators
+        self.waveforms = waveforms
+        self.rho0 = np.asarray(rho0, dtype=_f64)
+        self.obs = np.atleast_2d(np.asarray(obs, dtype=_f64))
+        if self.obs.shape[0] > _lib.MAX_OBS:
+            raise ValueError(f"at most {_lib.MAX_OBS} readout rows")
+        self.t0 = float(time_start)
+        self.t1 = float(time_end)
+        self.dt = float(time_step)
+        self.F = int(number_of_fine_divisions)
+        self.J = int(number_of_exponentials)
+        self.K = int(checkpoint_every)
+        if self.K < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.tolerance = tolerance
+        self.adjoint_tolerance = adjoint_tolerance
+        self.fd_rel = fd_rel
+        self.theta_scale = None if theta_scale is None else \
+            np.asarray(theta_scale, dtype=_f64)
+        self.n_out = int((self.t1 - self.t0) / self.dt)
+        self._cache = {}
+
+    # ------------------------------------------------------------------
+    def _fd_steps(self, theta):
+        scale = np.abs(theta)
+        if self.theta_scale is not None:
+            scale = np.maximum(scale, self.theta_scale)
+        scale = np.where(scale > 0, scale, 1.0)
+        return self.fd_rel * scale
+
+    def derivative_generators(self, theta):
+        """d generators / d theta_k, (n_theta, G, n, n), central FD."""
+        theta = np.asarray(theta, dtype=_f64)
+        steps = self._fd_steps(theta)
+        out = []
+        for k in range(theta.size):
+            tp = theta.copy()
+            tm = theta.copy()
+            tp[k] += steps[k]
+            tm[k] -= steps[k]
+            hk = tp[k] - tm[k]        # the step actually taken
+            gp = np.asarray(self.generators(tp), dtype=_f64)
+            gm = np.asarray(self.generators(tm), dtype=_f64)
+            out.append((gp - gm) / hk)
+        return np.array(out)
+
+    def setup(self, theta, amplitudes, derivatives=True):
+        """Plan, device tables and waveforms for one parameter point.
+
+        ``derivatives=False`` skips the derivative tables (forward only).
+        """
+        theta = np.asarray(theta, dtype=_f64).ravel()
+        amplitudes = np.asarray(amplitudes, dtype=_f64).ravel()
+        gens = np.asarray(self.generators(theta), dtype=_f64)
+        waves = self.waveforms(amplitudes)
+        sim = BatchSimulator(gens, self.J, self.F, tolerance=self.tolerance)
+        active, const, program = sim.program(waves)
+        prop = sim._propagator(active, const, self.dt)
+        h = self.dt / self.F
+        samples = sim.coefficient_box(waves, program)
+        m, _ = prop.plan(samples, h, tol=self.tolerance, max_substeps=1)
+        ma, _ = prop.plan(samples, h, tol=self.adjoint_tolerance,
+                          max_substeps=1, max_taylor=_lib.ADJ_MAX_TAYLOR)
+
+        # derivative tables of the folded constant matrix
+        dgen = self.derivative_generators(theta) \
+            if theta.size and derivatives else np.zeros((0,) + gens.shape)
+        varying = [g for g in program if g is not None]
+        for g in varying:
+            if np.any(dgen[:, g] != 0):
+                raise NotImplementedError(
+                    f"generator {g} has a time-dependent coefficient and "
+                    "depends on theta; only constant-coefficient generators "
+                    "may depend on parameters")
+        if const:
+            dconst = sum(c * dgen[:, g] for g, c in const.items())
+            theta_gen = 0
+        else:
+            if np.any(dgen != 0):
+                raise NotImplementedError("theta enters no constant generator")
+            dconst = np.zeros((dgen.shape[0],) + gens.shape[1:])
+            theta_gen = -1
+
+        kwaves = BatchSimulator.kernel_waves(waves, program)
+        tagged = sorted({k for k, w in enumerate(program)
+                         if any(p[0] >= 0 for wv in kwaves
+                                for p in wv.params[k])})
+        if len(tagged) > 1:
+            raise NotImplementedError(
+                "amplitude parameters on more than one generator")
+        grad_gen = tagged[0] if tagged else -1
+        recs, seg_arr, tag_index, tag_scale = waveform_records(
+            kwaves, len(program), with_tags=True)
+
+        perm = prop.perm
+        mats = prop._dense                                   # (M, n, n)
+        mp = mats[:, perm][:, :, perm]
+        dp = dconst[:, perm][:, :, perm]
+        stack = np.concatenate([np.transpose(mp, (0, 2, 1)),
+                                np.transpose(dp, (0, 2, 1))])
+        ellt = EllStack(stack, partner=True)
+        if ellt.slots > _lib.ADJ_MAX_SLOTS or \
+                prop.ell.slots > _lib.ADJ_MAX_SLOTS:
+            raise NotImplementedError(
+                f"adjoint patterns up to {_lib.ADJ_MAX_SLOTS} slots "
+                f"(forward {prop.ell.slots}, transposed {ellt.slots})")
+        M = mats.shape[0]
+        torch = _torch()
+        state = dict(
+            theta=theta, amplitudes=amplitudes, prop=prop, program=program,
+            taylor=m, taylor_adj=ma, theta_gen=theta_gen, grad_gen=grad_gen,
+            n_theta=dconst.shape[0] if theta_gen >= 0 else 0,
+            d_waves=torch.from_numpy(np.frombuffer(
+                recs.tobytes(), dtype=np.uint8).copy()).to("cuda"),
+            d_segs=device_tensor(seg_arr),
+            tag_index=tag_index, tag_scale=tag_scale,
+            n_records=seg_arr.shape[0],
+            d_col_t=device_tensor(ellt.col),
+            d_val_t=device_tensor(ellt.val[:M]),
+            d_dval_t=device_tensor(ellt.val[M:]) if dconst.shape[0] else
+            None,
+            slots_t=ellt.slots, n_inst=len(waves), dconst=dconst,
+            kernel_waves=kwaves)
+        return state
+
+    # ------------------------------------------------------------------
+    def forward(self, theta, amplitudes, state=None):
+        """(F (n_inst, n_obs) device tensor, state for ``backward``)."""
+        torch = _torch()
+        if state is None:
+            state = self.setup(theta, amplitudes)
+        prop = state["prop"]
+        n_inst = state["n_inst"]
+        rho0 = np.atleast_2d(self.rho0)
+        if rho0.shape[0] == 1 and n_inst > 1:
+            rho0 = np.repeat(rho0, n_inst, axis=0)
+        total = self.n_out * self.F
+        P = (total + self.K - 1) // self.K
+        ckpt = torch.empty((n_inst, P, prop.n), dtype=torch.float64,
+                           device="cuda")
+        out = prop.run(n_inst=n_inst, fine=self.F, n_out=self.n_out,
+                       h=self.dt / self.F, t0=self.t0, dt=self.dt,
+                       rho0=device_tensor(rho0), waves=state["d_waves"],
+                       segments=state["d_segs"], taylor=state["taylor"],
+                       substeps=1, read="integral", obs=self.obs,
+                       state_out=ckpt, ckpt_every=self.K)
+        state["ckpt"] = ckpt
+        state["n_ckpt"] = P
+        return out, state
+
+    def adjoint_desc(self, state):
+        prop = state["prop"]
+        return _lib.AdjointDesc(
+            n=prop.n, n_inst=state["n_inst"], n_gen=prop.K,
+            slots=prop.ell.slots, slots_t=state["slots_t"], n_exp=prop.J,
+            n_node=prop.S, fine=self.F, n_out=self.n_out,
+            taylor=int(state["taylor"]), taylor_adj=int(state["taylor_adj"]),
+            n_obs=self.obs.shape[0], flags=_lib.SSMX_COEF_WAVE, first_out=0,
+            partner=1, ckpt_every=self.K, n_theta=int(state["n_theta"]),
+            theta_gen=int(state["theta_gen"]),
+            grad_gen=int(state["grad_gen"]), reserved=0,
+            coef_inst_stride=0, h=self.dt / self.F, t0=self.t0, dt=self.dt)
+
+    def backward(self, state, seed, lambda_out=False):
+        """Per-instance gradients for readout seeds ``seed`` (n_inst, n_obs).
+
+        Returns a dict with ``theta`` (n_inst, n_theta), ``amplitudes``
+        (n_amplitudes,) summed over instances, ``records`` (per segment
+        record) and optionally ``lambda`` (n_inst, n), dLoss/d rho0 in the
+        standard basis.
+        """
+        torch = _torch()
+        prop = state["prop"]
+        n_inst = state["n_inst"]
+        desc = self.adjoint_desc(state)
+        lib = _lib.load()
+        nbytes = ctypes.c_uint64(0)
+        _lib.check(lib.ssmx_adjoint_workspace_bytes(ctypes.byref(desc),
+                                                   ctypes.byref(nbytes)))
+        work = torch.empty(int(nbytes.value) // 8, dtype=torch.float64,
+                           device="cuda")
+        seed = torch.as_tensor(seed, dtype=torch.float64).to("cuda")
+        seed = seed.reshape(n_inst, self.obs.shape[0]).contiguous()
+        rows = np.zeros((self.obs.shape[0], 64), dtype=_f64)
+        rows[:, :prop.n] = self.obs[:, prop.perm]
+        d_obs = device_tensor(rows)
+        n_theta = int(state["n_theta"])
+        g_theta = torch.zeros((n_inst, max(n_theta, 1)), dtype=torch.float64,
+                              device="cuda")
+        g_seg = torch.zeros(state["n_records"], dtype=torch.float64,
+                            device="cuda")
+        lam = torch.empty((n_inst, prop.n), dtype=torch.float64,
+                          device="cuda") if lambda_out else None
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = lib.ssmx_adjoint(
+            ctypes.byref(desc), _ptr(prop.d_col), _ptr(prop.d_val),
+            _ptr(state["d_col_t"]), _ptr(state["d_val_t"]),
+            _ptr(state["d_dval_t"]), _ptr(prop.d_w), _ptr(prop.d_nodes),
+            _ptr(state["d_waves"]), _ptr(state["d_segs"]),
+            _ptr(state["ckpt"]), _ptr(d_obs), _ptr(seed), _ptr(work),
+            _ptr(g_theta) if n_theta else None,
+            _ptr(g_seg) if state["grad_gen"] >= 0 else None,
+            _ptr(lam), ctypes.c_void_p(stream))
+        _lib.check(rc)
+        self._keep = (work, d_obs, seed)
+        result = dict(theta=g_theta[:, :n_theta], records=g_seg)
+        n_amp = state["amplitudes"].size
+        idx = state["tag_index"]
+        sel = np.nonzero(idx >= 0)[0]
+        g_amp = torch.zeros(n_amp, dtype=torch.float64, device="cuda")
+        if sel.size:
+            d_sel = torch.from_numpy(sel).to("cuda")
+            contrib = g_seg[d_sel] * torch.from_numpy(
+                state["tag_scale"][sel]).to("cuda")
+            g_amp.index_add_(0, torch.from_numpy(idx[sel]).to("cuda"),
+                             contrib)
+        result["amplitudes"] = g_amp
+        if lambda_out:
+            result["lambda"] = lam[:, prop.d_inv_perm]
+        return result
+
+    # ------------------------------------------------------------------
+    def readout(self, theta, amplitudes):
+        """Forward only: the integrated readouts (n_inst, n_obs)."""
+        state = self.setup(theta, amplitudes, derivatives=False)
+        out, _ = self.forward(theta, amplitudes, state)
+        return out
+
+
+class IntegratedReadout:
+    """``torch.autograd.Function`` over ``GradientProblem``.
+
+    ``IntegratedReadout.apply(theta, amplitudes, problem)`` returns the
+    (n_inst, n_obs) integrated readouts; gradients flow to ``theta`` and
+    ``amplitudes`` (float64 tensors on any device).
+    """
+
+    _fn = None
+
+    @classmethod
+    def function(cls):
+        if cls._fn is None:
+            torch = __import__("torch")
+
+            class _Fn(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, theta, amplitudes, problem):
+                    out, state = problem.forward(
+                        theta.detach().cpu().numpy(),
+                        amplitudes.detach().cpu().numpy())
+                    ctx.problem = problem
+                    ctx.state = state
+                    ctx.devices = (theta.device, amplitudes.device)
+                    return out
+
+                @staticmethod
+                def backward(ctx, grad_out):
+                    res = ctx.problem.backward(ctx.state, grad_out)
+                    gt = res["theta"].sum(dim=0).to(ctx.devices[0])
+                    ga = res["amplitudes"].to(ctx.devices[1])
+                    return gt, ga, None
+
+            cls._fn = _Fn
+        return cls._fn
+
+    @classmethod
+    def apply(cls, theta, amplitudes, problem):
+        return cls.function().apply(theta, amplitudes, problem)
+
+
+def step_flops_adjoint(state, slots_nnz_t):
+    """Algorithmic FP64 FLOPs of one backward step of one instance.
+
+    Recomputation: J (2 K nnz + m 2 (nnz + n)); adjoint: J [(m_a - 1)
+    (2 nnz + 2 n) forward terms + m_a (2 nnz_t + 2 n) adjoint terms +
+    m_a 2 nnz_t Frechet accumulation + m_a^2 n coefficient sums].
+    """
+    prop = state["prop"]
+    J, n = prop.J, prop.n
+    nnz = prop.ell.nnz
+    nnz_t = slots_nnz_t
+    m, ma = state["taylor"], state["taylor_adj"]
+    K = prop.K
+    recompute = J * (2 * K * nnz + m * 2 * (nnz + n))
+    adj = J * ((ma - 1) * (2 * nnz + 2 * n) + ma * (2 * nnz_t + 2 * n)
+               + ma * 2 * nnz_t + ma * ma * n + 2 * K * (nnz + nnz_t))
+    return recompute + adj
+
+
+__all__ = ["GradientProblem", "IntegratedReadout"]

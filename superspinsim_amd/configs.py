@@ -14,6 +14,14 @@
   (coherent) vectorisation, n = 49 (``use_incoherent=False``); n = 19 with
   the compressed one.
 
+* ``odmr7_gradient`` - the ``odmr7`` sweep (n = 49) with a piecewise-
+  constant microwave amplitude (200 segments of 10 ns, Rabi frequencies
+  drawn N(3 MHz, 0.5 MHz), seed 3) as a ``GradientProblem`` over the
+  physical parameters ``THETA_NAMES`` (ground and excited zero-field
+  splittings, static field Bx/By/Bz, radiative rate, excited ms = +-1 ISC
+  rate, ground T1 and T2*) and the 200 amplitudes; the readout is the
+  integrated fluorescence, checkpoints every 1000 steps.
+
 Drive amplitude convention (the reference docs' ``examples.rst``):
 ``B1 = sqrt(2) * Omega / 28 GHz/T`` for a Rabi frequency Omega (Hz).
 """
@@ -24,6 +32,7 @@ import math
 import numpy as np
 
 from superspinsim_amd import models, params
+from superspinsim_amd.adjoint import GradientProblem
 from superspinsim_amd.generate_generators import generate_atoms
 from superspinsim_amd.sweep import BatchSimulator, Waveform
 
@@ -132,3 +141,98 @@ def odmr7(count=4096, use_incoherent=False):
         3e6, 5e6 / 65e6, 2e-6, 2e-9, 100, obs_levels=[3, 4, 5],
         rho0_diag={0: 1 / 3, 1: 1 / 3, 2: 1 / 3},
         use_incoherent=use_incoherent)
+
+
+# ---------------------------------------------------------------------------
+# Parametrised 7-level model for gradients
+# ---------------------------------------------------------------------------
+
+THETA_NAMES = ("D_gs", "D_es", "Bx", "By", "Bz", "radiative", "isc_pm",
+               "T1", "T2star")
+#: scale of each parameter for finite-difference steps where it is 0
+THETA_SCALE = np.array([2.87e9, 1.42e9, 5e-3, 5e-3, 5e-3, 65e6, 50e6, 1e-2,
+                        1e-4])
+
+
+def nv7_theta0():
+    """Parameter point of ``odmr7`` in ``THETA_NAMES`` order."""
+    ground = params.nv.room.ground
+    return np.array([2.87e9, 1.42e9, 0.0, 0.0, 5e-3, 65e6, 50e6,
+                     ground.thermalisation_time, ground.dephasing_time])
+
+
+def nv7_generators(theta, use_incoherent=False):
+    """[L0, Gx, Gy, Gz, Gr] of the 7-level model at ``theta``."""
+    d_gs, d_es, bx, by, bz, rel, isc, t1, t2 = [float(v) for v in theta]
+    spins, spin_int, groups = _nv7_description(bz)
+    field = np.array([bx, by, bz])
+    spins[0][0]["D"] = math.tau * d_gs
+    spins[1][0]["D"] = math.tau * d_es
+    spins[0][0]["B0"] = field
+    spins[1][0]["B0"] = field.copy()
+    spins[0][0]["TS1"] = t1
+    spins[0][0]["TS2"] = t2
+    groups[((0, 0), (1, 0))] = {"rel": rel}
+    groups[((1, 0), (2, 0))] = {"s_gets_0": 8e6, "s_gets_1": isc}
+    ops, _ = generate_atoms(spins, spin_int, groups,
+                            use_incoherent=use_incoherent)
+    g = ops["generators"]
+    return np.array([g["L0"], g["Gx"], g["Gy"], g["Gz"], g["Gr"]])
+
+
+class GradientWorkload:
+    """Frozen gradient workload: problem, parameter point, loss target."""
+
+    def __init__(self, name, problem, theta, amplitudes, frequencies, seed):
+        self.name = name
+        self.problem = problem
+        self.theta = theta
+        self.amplitudes = amplitudes
+        self.frequencies = frequencies
+        self.seed = seed
+        rng = np.random.default_rng(seed + 1000)
+        #: parameters the loss target is simulated at (+-2 %)
+        self.theta_target = theta * (1 + rng.uniform(-0.02, 0.02,
+                                                     theta.size))
+        self.amplitudes_target = amplitudes * (
+            1 + rng.uniform(-0.02, 0.02, amplitudes.size))
+
+
+def odmr7_gradient(count=1024, segments=200, seed=3, time_end=2e-6,
+                   time_step=2e-9, fine=100, checkpoint_every=1000,
+                   use_incoherent=False, tolerance=1e-16,
+                   adjoint_tolerance=1e-13, frequencies=None):
+    """C1-style ODMR sweep with segment amplitudes, as a GradientProblem.
+
+    ``frequencies`` overrides the 2.77-2.97 GHz sweep (``count`` points).
+    """
+    rng = np.random.default_rng(seed)
+    amplitudes = rng.normal(3e6, 0.5e6, segments)
+    freqs = np.linspace(2.77e9, 2.97e9, count) if frequencies is None \
+        else np.asarray(frequencies, dtype=np.float64)
+    base = odmr7(1, use_incoherent=use_incoherent)
+    seg_len = time_end / segments
+    dscale = drive_amplitude(1.0)
+
+    def waveforms(amps):
+        out = []
+        for f in freqs:
+            w = Waveform(5, base=[1.0, 0.0, 0.0, 0.0, base.excitation])
+            for k in range(segments):
+                w.add(1, k * seg_len, (k + 1) * seg_len,
+                      drive_amplitude(amps[k]), math.tau * f, 0.0,
+                      param=k, dparam=dscale)
+            out.append(w)
+        return out
+
+    def generators(theta):
+        return nv7_generators(theta, use_incoherent=use_incoherent)
+
+    problem = GradientProblem(
+        generators, waveforms, base.rho0, base.obs, 0.0, time_end,
+        time_step, fine, 2, checkpoint_every, tolerance=tolerance,
+        adjoint_tolerance=adjoint_tolerance, theta_scale=THETA_SCALE)
+    name = "odmr7-gradient-n49" if not use_incoherent else \
+        "odmr7-gradient-n19"
+    return GradientWorkload(name, problem, nv7_theta0(), amplitudes, freqs,
+                            seed)

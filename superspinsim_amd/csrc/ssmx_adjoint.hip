@@ -1,0 +1,787 @@
+// ssmx_adjoint.hip - reverse-mode gradient of the integrated readout of the
+// batched commutator-free Magnus propagator (gfx950).
+//
+// Forward (ssmx_propagate, non-rotating frame, substeps = 1): per step k,
+//   x_{k+1} = E_{k,0} E_{k,1} ... E_{k,J-1} x_k,  E_{kj} = T_m(A_kj),
+// T_m the degree-m Taylor polynomial, A_kj = sum_M f_{kj,M} G_M, and the
+// readout F = sum_k w_k obs . x_k (trapezoid weights).  The adjoint runs the
+// steps backwards:
+//   lambda_T = (h/2) s,   lambda_k = Phi_k^T lambda_{k+1} + w_k s,
+// with s = sum_o seed_o obs_o, and for each exponential (input y, output
+// cotangent mu) accumulates the exponent gradient
+//   Abar = sum_{i + t < m_a} c_it u_i w_t^T,  u_i = (A^T)^i mu / i!,
+//   w_t = A^t y / t!,  c_it = i! t! / (i + t + 1)!,
+// which is the adjoint action of the block-triangular augmented exponential
+// exp([[A, E], [0, A]]) whose upper block is the Frechet derivative of exp
+// at A in direction E: the powers of A (forward terms w_t) and of A^T
+// (adjoint terms u_i) are shared between the exponential and its
+// derivative.  mu is then mapped through T_{m_a}(A)^T = sum_i u_i.
+//
+// Work split: one workgroup of two wavefronts per instance.  Wavefront 0
+// (the recompute wavefront) re-runs segment p - 1 of ckpt_every steps from
+// its checkpoint and stores every exponential's input y into the
+// workspace, while wavefront 1 (the adjoint wavefront) runs the adjoint of
+// segment p, reading those inputs in reverse; a workgroup barrier ends each
+// phase.  Inside the adjoint wavefront the forward terms w_t of the next
+// exponential are computed in the same loop as the adjoint terms u_i of the
+// current one: two independent dependency chains per lane.
+//
+// Abar is accumulated on a transposed ELL pattern (lane c, slot -> row r):
+// entry Abar[r][c] = sum_i u_i[r] q_i[c] with q_i = sum_t c_it w_t, so the
+// gathers of u_i that the product A^T u_i needs also feed the accumulation.
+
+#include "ssmx_common.h"
+
+namespace {
+
+using ssmx::kWave;
+using ssmx::node_time;
+using ssmx::read_lane;
+using ssmx::set_error;
+using ssmx::swap_pair;
+using ssmx::wave_coef;
+using ssmx::wave_coef_seek;
+using ssmx::wave_exchange;
+using ssmx::wave_sum;
+
+constexpr int kMaxAdj = SSMX_ADJ_MAX_TAYLOR;
+constexpr int kMaxSlots = 32;
+
+// 1/q, q = 0..40 (entry 0 unused); the recompute uses the same table and
+// recursion as the propagator so that it reproduces the forward states.
+__constant__ double kInvA[SSMX_MAX_TAYLOR + 1] = {
+    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
+    1.0 / 6,    1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
+    1.0 / 12,   1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
+    1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
+    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
+    1.0 / 30,   1.0 / 31,   1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35,
+    1.0 / 36,   1.0 / 37,   1.0 / 38, 1.0 / 39, 1.0 / 40};
+
+struct AdjParams {
+    ssmx_adjoint_desc d;
+    const int32_t *col_f;
+    const double *val_f;
+    const int32_t *col_t;
+    const double *val_t;
+    const double *dval_t;
+    const double *w;
+    const double *nodes;
+    const double *coef;
+    const ssmx_wave *waves;
+    const ssmx_segment *segs;
+    const double *ckpt;
+    const double *obs;
+    const double *seed;
+    double *work;
+    double *grad_theta;
+    double *grad_seg;
+    double *lambda_out;
+};
+
+// Scalar factor of matrix m in exponential j of global step `step`
+// (non-rotating): h * sum_s w[j,s] c_m(t_s), in the propagator's order.
+// `seek` selects the two-way segment cursor.
+template <bool SEEK>
+__device__ __forceinline__ double step_factor(const AdjParams &p, const double *coef_i,
+                              const ssmx_wave *wv, long step, int j, int m,
+                              int &cursor)
+{
+    const ssmx_adjoint_desc &d = p.d;
+    const int S = d.n_node, K = d.n_gen;
+    const long i_loc = step / d.fine;
+    const int r = (int)(step - i_loc * d.fine);
+    const long ig = (long)d.first_out + i_loc;
+    double cw = 0.0;
+    if (wv) {
+        for (int s = 0; s < S; ++s) {
+            const double t = node_time(d.t0, d.dt, d.fine, r, ig, p.nodes[s]);
+            double c;
+            if (SEEK) {
+                int rec;
+                double damp;
+                c = wave_coef_seek(wv, p.segs, m, t, cursor, rec, damp);
+            } else {
+                c = wave_coef(wv, p.segs, m, t, cursor);
+            }
+            cw = fma(p.w[j * S + s], c, cw);
+        }
+    } else {
+        const double *row = coef_i + ((size_t)(i_loc * d.fine + r) * S) * K;
+        for (int s = 0; s < S; ++s)
+            cw = fma(p.w[j * S + s], row[s * K + m], cw);
+    }
+    return d.h * cw;
+}
+
+// Segment record of generator g holding node s of global step `st` (or -1)
+// and d c_g / d amp there.
+__device__ __forceinline__ double amp_seek(const AdjParams &p,
+                                        const ssmx_wave *wv, long st, int s,
+                                        int g, int &cursor, int &rec)
+{
+    const ssmx_adjoint_desc &d = p.d;
+    const long i_loc = st / d.fine;
+    const int r = (int)(st - i_loc * d.fine);
+    const double t = node_time(d.t0, d.dt, d.fine, r,
+                               (long)d.first_out + i_loc, p.nodes[s]);
+    double damp;
+    wave_coef_seek(wv, p.segs, g, t, cursor, rec, damp);
+    return damp;
+}
+
+// y = A v on an ELL pattern (slot 0 = own lane, slot 1 = partner lane
+// r ^ 1 by DPP or a gathered column, slots 2.. gathered through `xbuf`).
+// Same order of operations as the propagator's product.
+template <int SL>
+__device__ __forceinline__ double ell_product(const double (&a)[SL],
+                                              const int (&col)[SL], double v,
+                                              double *xbuf, int lane,
+                                              bool use_partner)
+{
+    double g[SL];
+    g[0] = v;
+    xbuf[lane] = v;
+    wave_exchange();
+    if (SL > 1)
+        g[1] = use_partner ? swap_pair(v) : xbuf[col[1]];
+#pragma unroll
+    for (int sl = 2; sl < SL; ++sl)
+        g[sl] = xbuf[col[sl]];
+    wave_exchange();
+    double y0 = a[0] * g[0];
+    double y1 = 0.0;
+#pragma unroll
+    for (int sl = 1; sl < SL; ++sl) {
+        if (sl & 1)
+            y1 = fma(a[sl], g[sl], y1);
+        else
+            y0 = fma(a[sl], g[sl], y0);
+    }
+    return y0 + y1;
+}
+
+// a[sl] = sum_m f_m V_m[sl] with f_m broadcast from lane base + m of
+// `factor` and V staged in LDS as [m][slot / 2][lane][slot % 2].
+template <int SL>
+__device__ __forceinline__ void assemble(double (&a)[SL], const double *lv,
+                                         double factor, int base, int M,
+                                         int lane)
+{
+    constexpr int SLP = SL + (SL & 1);
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl)
+        a[sl] = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const double f = read_lane(factor, base + m);
+        const double2 *v =
+            reinterpret_cast<const double2 *>(lv + m * SLP * kWave) + lane;
+#pragma unroll
+        for (int pr = 0; pr < SLP / 2; ++pr) {
+            const double2 vv = v[pr * kWave];
+            a[2 * pr] = fma(f, vv.x, a[2 * pr]);
+            if (2 * pr + 1 < SL)
+                a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
+        }
+    }
+}
+
+// Patterns up to 10 slots run two wavefronts per SIMD (256 VGPRs each);
+// wider ones one.
+template <int SF, int ST>
+__global__ __attribute__((amdgpu_flat_work_group_size(2 * kWave, 2 * kWave),
+                          amdgpu_waves_per_eu(ST <= 10 ? 2 : 1))) void
+adjoint_kernel(AdjParams p)
+{
+    constexpr int SFP = SF + (SF & 1);
+    constexpr int STP = ST + (ST & 1);
+    constexpr int MA = kMaxAdj;
+    extern __shared__ double lds[];
+    const ssmx_adjoint_desc &d = p.d;
+    const int M = d.n_gen;
+    // exchange buffers at LDS offset 0 (the buffer is an immediate offset
+    // of the gather instructions), then forward and transposed values
+    double *xb = lds;
+    double *lf = lds + 3 * kWave;
+    double *lt = lf + M * SFP * kWave;
+
+    for (int e = threadIdx.x; e < M * SFP * kWave; e += blockDim.x) {
+        const int half = e & 1;
+        const int row = (e >> 1) % kWave;
+        const int pair = (e / (2 * kWave)) % (SFP / 2);
+        const int m = e / (kWave * SFP);
+        const int slot = 2 * pair + half;
+        lf[e] = slot < d.slots
+                    ? p.val_f[((size_t)m * d.slots + slot) * kWave + row]
+                    : 0.0;
+    }
+    for (int e = threadIdx.x; e < M * STP * kWave; e += blockDim.x) {
+        const int half = e & 1;
+        const int row = (e >> 1) % kWave;
+        const int pair = (e / (2 * kWave)) % (STP / 2);
+        const int m = e / (kWave * STP);
+        const int slot = 2 * pair + half;
+        lt[e] = slot < d.slots_t
+                    ? p.val_t[((size_t)m * d.slots_t + slot) * kWave + row]
+                    : 0.0;
+    }
+    __syncthreads();
+
+    const int inst = blockIdx.x;
+    const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n = d.n;
+    const int J = d.n_exp;
+    const int S = d.n_node;
+    const bool use_partner = d.partner != 0;
+    const long total = (long)d.n_out * d.fine;
+    const int K = d.ckpt_every;
+    const int P = (int)((total + K - 1) / K);
+    const bool wave_mode = (d.flags & SSMX_COEF_WAVE) != 0;
+    const double *coef_i =
+        wave_mode ? nullptr : p.coef + (size_t)inst * d.coef_inst_stride;
+    const ssmx_wave *wv = wave_mode ? p.waves + inst : nullptr;
+    const int grad_gen = p.grad_seg ? d.grad_gen : -1;
+    const int theta_gen = d.theta_gen;
+    const int ma = d.taylor_adj;
+
+    int cf[SF];
+    cf[0] = lane;
+#pragma unroll
+    for (int sl = 1; sl < SF; ++sl)
+        cf[sl] = sl < d.slots ? p.col_f[sl * kWave + lane] : lane;
+    const double inv_lane = kInvA[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+
+    // Both wavefronts pass the same number of workgroup barriers (P + 1 in
+    // the loops below, plus the staging barrier above); the branch on the
+    // wavefront's role is uniform within each wavefront.
+    if (role == 0) {
+        for (int q = 0; q <= P; ++q) {
+            // ---------------- recompute segment P - 1 - q ----------------
+            if (q < P) {
+                const int seg = P - 1 - q;
+                const long k0 = (long)seg * K;
+                const long k1 = k0 + K < total ? k0 + K : total;
+                double *wb = p.work + ((size_t)inst * 2 + (seg & 1)) *
+                                          (size_t)K * J * kWave;
+                double *xr = xb;
+                double x = lane < n ? p.ckpt[((size_t)inst * P + seg) * n +
+                                             lane]
+                                    : 0.0;
+                const int JM = J * M;
+                const int B = kWave / JM;
+                const int my_b = lane / JM;
+                const int my_j = (lane % JM) / M;
+                const int my_m = lane % M;
+                int cursor = 0;
+                double factor = 0.0;
+                int b = 0;
+                for (long k = k0; k < k1; ++k, ++b) {
+                    if (b == B)
+                        b = 0;
+                    if (b == 0) {
+                        const long mine = k + my_b;
+                        factor = (my_b < B && mine < k1)
+                                     ? step_factor<false>(p, coef_i, wv, mine,
+                                                          my_j, my_m, cursor)
+                                     : 0.0;
+                    }
+                    for (int j = J - 1; j >= 0; --j) {
+                        wb[((size_t)(k - k0) * J + j) * kWave + lane] = x;
+                        double a[SF];
+                        assemble<SF>(a, lf, factor, (b * J + j) * M, M, lane);
+                        double term = x;
+                        double acc = x;
+                        for (int t = 1; t <= d.taylor; ++t) {
+                            term = ell_product<SF>(a, cf, term, xr, lane,
+                                                   use_partner) *
+                                   read_lane(inv_lane, t);
+                            acc += term;
+                        }
+                        x = acc;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+
+    // ---- adjoint wavefront state (persists over segments) ----
+    int ct[ST];
+    ct[0] = lane;
+#pragma unroll
+    for (int sl = 1; sl < ST; ++sl)
+        ct[sl] = sl < d.slots_t ? p.col_t[sl * kWave + lane] : lane;
+    double seedrow = 0.0;
+    for (int o = 0; o < d.n_obs; ++o)
+        seedrow = fma(p.seed[(size_t)inst * d.n_obs + o], p.obs[o * kWave + lane],
+                      seedrow);
+    double lam = (0.5 * d.h) * seedrow;
+    double Sd[ST];
+#pragma unroll
+    for (int sl = 0; sl < ST; ++sl)
+        Sd[sl] = 0.0;
+    int cur_rec = -1;
+    int cursor_a = 0;
+    const double *lt_grad = lt + (grad_gen >= 0 ? grad_gen : 0) * STP * kWave;
+
+    // Amplitude gradient of the current segment record: per-lane partial
+    // sums of <Abar, G_grad_gen>, reduced over the wavefront only when the
+    // record changes.
+    double amp_acc = 0.0;
+    auto flush = [&]() __attribute__((always_inline)) {
+        if (cur_rec >= 0) {
+            const double v = wave_sum(amp_acc);
+            if (lane == 0)
+                p.grad_seg[cur_rec] += v;
+            amp_acc = 0.0;
+        }
+    };
+
+    for (int q = 0; q <= P; ++q) {
+        if (q > 0) {
+            // ---------------- adjoint of segment P - q ----------------
+            const int seg = P - q;
+            const long k0 = (long)seg * K;
+            const long k1 = k0 + K < total ? k0 + K : total;
+            const double *wb = p.work + ((size_t)inst * 2 + (seg & 1)) *
+                                            (size_t)K * J * kWave;
+            double *xu = xb + kWave;
+            double *xv = xb + 2 * kWave;
+            const long E = (k1 - k0) * J;
+            const int JM = J * M;
+            const int WB = JM + S;          // batch lanes per step
+            const int BA = kWave / WB;      // steps per batch
+            const int my_b = lane / WB;
+            const int my_r = lane % WB;
+            long kb = 0;                    // batch holds kb, kb - 1, ...
+            double fac = 0.0, damp_l = 0.0;
+            int rec_l = -1;
+
+            auto load_batch = [&](long top) __attribute__((always_inline)) {
+                kb = top;
+                const long st = top - my_b;
+                fac = 0.0;
+                damp_l = 0.0;
+                rec_l = -1;
+                if (my_b < BA && st >= k0) {
+                    if (my_r < JM) {
+                        fac = step_factor<true>(p, coef_i, wv, st, my_r / M,
+                                                my_r % M, cursor_a);
+                    } else if (grad_gen >= 0) {
+                        damp_l = amp_seek(p, wv, st, my_r - JM, grad_gen,
+                                          cursor_a, rec_l);
+                    }
+                }
+            };
+            auto y_at = [&](long idx) __attribute__((always_inline)) -> double {
+                const long k = k1 - 1 - idx / J;
+                const int j = (int)(idx % J);
+                return wb[((size_t)(k - k0) * J + j) * kWave + lane];
+            };
+
+            // current exponential: step ck, exponential cj
+            long ck = k1 - 1;
+            int cj = 0;
+            load_batch(ck);
+            double a_f[SF], a_t[ST];
+            int base = (int)(kb - ck) * WB;
+            assemble<SF>(a_f, lf, fac, base + cj * M, M, lane);
+            assemble<ST>(a_t, lt, fac, base + cj * M, M, lane);
+            double f_dep = theta_gen >= 0
+                               ? read_lane(fac, base + cj * M + theta_gen)
+                               : 0.0;
+            int rec_s[SSMX_MAX_NODE];
+            double dw_s[SSMX_MAX_NODE];
+            auto extract_amp = [&](int bse, int j) __attribute__((always_inline)) {
+#pragma unroll
+                for (int s = 0; s < SSMX_MAX_NODE; ++s) {
+                    rec_s[s] = -1;
+                    dw_s[s] = 0.0;
+                    if (grad_gen >= 0 && s < S) {
+                        rec_s[s] = __builtin_amdgcn_readlane(rec_l,
+                                                             bse + JM + s);
+                        dw_s[s] = (d.h * p.w[j * S + s]) *
+                                  read_lane(damp_l, bse + JM + s);
+                    }
+                }
+            };
+            extract_amp(base, cj);
+
+            // forward terms of the first exponential
+            double W[MA];
+#pragma unroll
+            for (int t = 0; t < MA; ++t)
+                W[t] = 0.0;
+            W[0] = y_at(0);
+#pragma unroll
+            for (int t = 1; t < MA; ++t) {
+                if (t < ma)
+                    W[t] = ell_product<SF>(a_f, cf, W[t - 1], xv, lane,
+                                           use_partner) *
+                           (1.0 / t);
+            }
+            double ypre = E > 1 ? y_at(1) : 0.0;
+
+            for (long idx = 0; idx < E; ++idx) {
+                // q_i = sum_t c_it w_t with c_i0 = 1 / (i + 1) and
+                // c_it = c_i,t-1 * t / (i + t + 1); the loops are unrolled,
+                // so every c_it is a compile-time constant.  Terms with
+                // t >= m_a vanish (W[t] = 0 there).
+                double Q[MA];
+#pragma unroll
+                for (int i = 0; i < MA; ++i) {
+                    double c = 1.0 / (i + 1);
+                    double qv = c * W[0];
+#pragma unroll
+                    for (int t = 1; t + i < MA; ++t) {
+                        c = c * t / (i + t + 1);
+                        qv = fma(c, W[t], qv);
+                    }
+                    Q[i] = qv;
+                }
+                const bool has_next = idx + 1 < E;
+                long nk = ck;
+                int nj = cj + 1;
+                if (nj == J) {
+                    nj = 0;
+                    --nk;
+                }
+                if (has_next) {
+                    if (nk <= kb - BA)
+                        load_batch(nk);
+                    const int nbase = (int)(kb - nk) * WB;
+                    assemble<SF>(a_f, lf, fac, nbase + nj * M, M, lane);
+#pragma unroll
+                    for (int t = 0; t < MA; ++t)
+                        W[t] = 0.0;
+                    W[0] = ypre;
+                    ypre = idx + 2 < E ? y_at(idx + 2) : 0.0;
+                }
+
+                double u = lam;
+                double lam_new = lam;
+                double T[ST];
+#pragma unroll
+                for (int sl = 0; sl < ST; ++sl)
+                    T[sl] = 0.0;
+#pragma unroll
+                for (int i = 0; i < MA; ++i) {
+                    if (i >= ma)
+                        continue;
+                    const bool fwd = has_next && i + 1 < ma;
+                    xu[lane] = u;
+                    if (fwd)
+                        xv[lane] = W[i];
+                    wave_exchange();
+                    double gu[ST];
+                    gu[0] = u;
+                    gu[1] = use_partner ? swap_pair(u) : xu[ct[1]];
+#pragma unroll
+                    for (int sl = 2; sl < ST; ++sl)
+                        gu[sl] = xu[ct[sl]];
+                    double gv[SF];
+                    if (fwd) {
+                        gv[0] = W[i];
+                        gv[1] = use_partner ? swap_pair(W[i]) : xv[cf[1]];
+#pragma unroll
+                        for (int sl = 2; sl < SF; ++sl)
+                            gv[sl] = xv[cf[sl]];
+                    }
+                    wave_exchange();
+#pragma unroll
+                    for (int sl = 0; sl < ST; ++sl)
+                        T[sl] = fma(gu[sl], Q[i], T[sl]);
+                    double u0 = a_t[0] * gu[0];
+                    double u1 = 0.0;
+#pragma unroll
+                    for (int sl = 1; sl < ST; ++sl) {
+                        if (sl & 1)
+                            u1 = fma(a_t[sl], gu[sl], u1);
+                        else
+                            u0 = fma(a_t[sl], gu[sl], u0);
+                    }
+                    u = (u0 + u1) * (1.0 / (i + 1));
+                    lam_new += u;
+                    if (fwd) {
+                        double v0 = a_f[0] * gv[0];
+                        double v1 = 0.0;
+#pragma unroll
+                        for (int sl = 1; sl < SF; ++sl) {
+                            if (sl & 1)
+                                v1 = fma(a_f[sl], gv[sl], v1);
+                            else
+                                v0 = fma(a_f[sl], gv[sl], v0);
+                        }
+                        W[i + 1 < MA ? i + 1 : MA - 1] =
+                            (v0 + v1) * (1.0 / (i + 1));
+                    }
+                }
+                lam = lam_new;
+
+                // exponent gradient -> accumulators
+                if (theta_gen >= 0) {
+#pragma unroll
+                    for (int sl = 0; sl < ST; ++sl)
+                        Sd[sl] = fma(f_dep, T[sl], Sd[sl]);
+                }
+                if (grad_gen >= 0) {
+                    // <Abar, G_grad_gen> restricted to this lane's column
+                    double tg = 0.0;
+                    const double2 *g2 =
+                        reinterpret_cast<const double2 *>(lt_grad) + lane;
+#pragma unroll
+                    for (int pr = 0; pr < STP / 2; ++pr) {
+                        const double2 gg = g2[pr * kWave];
+                        tg = fma(T[2 * pr], gg.x, tg);
+                        if (2 * pr + 1 < ST)
+                            tg = fma(T[2 * pr + 1], gg.y, tg);
+                    }
+#pragma unroll
+                    for (int s = 0; s < SSMX_MAX_NODE; ++s) {
+                        if (s < S && rec_s[s] >= 0) {
+                            if (rec_s[s] != cur_rec) {
+                                flush();
+                                cur_rec = rec_s[s];
+                            }
+                            amp_acc = fma(dw_s[s], tg, amp_acc);
+                        }
+                    }
+                }
+                if (cj == J - 1)
+                    lam = fma(ck == 0 ? 0.5 * d.h : d.h, seedrow, lam);
+
+                if (has_next) {
+                    const int nbase = (int)(kb - nk) * WB;
+                    assemble<ST>(a_t, lt, fac, nbase + nj * M, M, lane);
+                    f_dep = theta_gen >= 0
+                                ? read_lane(fac, nbase + nj * M + theta_gen)
+                                : 0.0;
+                    extract_amp(nbase, nj);
+                    ck = nk;
+                    cj = nj;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    {
+        flush();
+        if (p.grad_theta) {
+            for (int t = 0; t < d.n_theta; ++t) {
+                double v = 0.0;
+#pragma unroll
+                for (int sl = 0; sl < ST; ++sl)
+                    if (sl < d.slots_t)
+                        v = fma(Sd[sl],
+                                p.dval_t[((size_t)t * d.slots_t + sl) * kWave +
+                                         lane],
+                                v);
+                v = wave_sum(v);
+                if (lane == 0)
+                    p.grad_theta[(size_t)inst * d.n_theta + t] = v;
+            }
+        }
+        if (p.lambda_out && lane < n)
+            p.lambda_out[(size_t)inst * n + lane] = lam;
+    }
+}
+
+template <int SF, int ST>
+int launch(const AdjParams &p, hipStream_t stream)
+{
+    constexpr int SFP = SF + (SF & 1);
+    constexpr int STP = ST + (ST & 1);
+    const int M = p.d.n_gen;
+    const size_t lds =
+        ((size_t)M * (SFP + STP) * kWave + 3 * kWave) * sizeof(double);
+    if (lds > 160 * 1024)
+        return set_error(SSMX_EINVAL,
+                         "adjoint tables need %zu bytes of LDS (> 160 KiB)",
+                         lds);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&adjoint_kernel<SF, ST>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
+                             hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL((adjoint_kernel<SF, ST>), dim3(p.d.n_inst),
+                       dim3(2 * kWave), lds, stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(SSMX_ELAUNCH, "adjoint launch: %s",
+                         hipGetErrorString(e));
+    return SSMX_OK;
+}
+
+// (forward, transposed) slot counts with a compiled kernel, cheapest first;
+// a problem runs on the first pair that holds both patterns.
+constexpr int kPairs[][2] = {{4, 4},   {4, 7},   {7, 7},   {4, 10},
+                             {7, 10},  {10, 10}, {7, 16},  {10, 16},
+                             {16, 16}, {16, 24}, {24, 24}, {32, 32}};
+
+int dispatch(const AdjParams &p, hipStream_t s)
+{
+    int pick = -1;
+    for (int i = 0; i < (int)(sizeof(kPairs) / sizeof(kPairs[0])); ++i)
+        if (p.d.slots <= kPairs[i][0] && p.d.slots_t <= kPairs[i][1]) {
+            pick = i;
+            break;
+        }
+    switch (pick) {
+    case 0: return launch<4, 4>(p, s);
+    case 1: return launch<4, 7>(p, s);
+    case 2: return launch<7, 7>(p, s);
+    case 3: return launch<4, 10>(p, s);
+    case 4: return launch<7, 10>(p, s);
+    case 5: return launch<10, 10>(p, s);
+    case 6: return launch<7, 16>(p, s);
+    case 7: return launch<10, 16>(p, s);
+    case 8: return launch<16, 16>(p, s);
+    case 9: return launch<16, 24>(p, s);
+    case 10: return launch<24, 24>(p, s);
+    case 11: return launch<32, 32>(p, s);
+    default: break;
+    }
+    return set_error(SSMX_EINVAL, "no adjoint kernel for slots %d / %d",
+                     p.d.slots, p.d.slots_t);
+}
+
+int validate(const ssmx_adjoint_desc *d)
+{
+    if (!d)
+        return set_error(SSMX_EINVAL, "null descriptor");
+    if (d->n < 1 || d->n > SSMX_MAX_N)
+        return set_error(SSMX_EINVAL, "n=%d outside 1..%d", d->n, SSMX_MAX_N);
+    if (d->n_inst < 1)
+        return set_error(SSMX_EINVAL, "n_inst=%d < 1", d->n_inst);
+    if (d->n_gen < 1 || d->n_gen > SSMX_MAX_GEN)
+        return set_error(SSMX_EINVAL, "n_gen=%d outside 1..%d", d->n_gen,
+                         SSMX_MAX_GEN);
+    if (d->slots < 1 || d->slots > kMaxSlots)
+        return set_error(SSMX_EINVAL, "slots=%d outside 1..%d", d->slots,
+                         kMaxSlots);
+    if (d->slots_t < 1 || d->slots_t > kMaxSlots)
+        return set_error(SSMX_EINVAL, "slots_t=%d outside 1..%d", d->slots_t,
+                         kMaxSlots);
+    if (d->n_exp < 1 || d->n_exp > SSMX_MAX_EXP)
+        return set_error(SSMX_EINVAL, "n_exp=%d outside 1..%d", d->n_exp,
+                         SSMX_MAX_EXP);
+    if (d->n_node < 1 || d->n_node > SSMX_MAX_NODE)
+        return set_error(SSMX_EINVAL, "n_node=%d outside 1..%d", d->n_node,
+                         SSMX_MAX_NODE);
+    if (d->n_exp * d->n_gen > kWave ||
+        d->n_exp * d->n_gen + d->n_node > kWave)
+        return set_error(SSMX_EINVAL,
+                         "n_exp * n_gen + n_node = %d exceeds one batch",
+                         d->n_exp * d->n_gen + d->n_node);
+    if (d->fine < 1 || d->n_out < 1)
+        return set_error(SSMX_EINVAL, "fine=%d, n_out=%d", d->fine, d->n_out);
+    if (d->taylor < 1 || d->taylor > SSMX_MAX_TAYLOR)
+        return set_error(SSMX_EINVAL, "taylor=%d outside 1..%d", d->taylor,
+                         SSMX_MAX_TAYLOR);
+    if (d->taylor_adj < 1 || d->taylor_adj > SSMX_ADJ_MAX_TAYLOR)
+        return set_error(SSMX_EINVAL, "taylor_adj=%d outside 1..%d",
+                         d->taylor_adj, SSMX_ADJ_MAX_TAYLOR);
+    if (d->n_obs < 1 || d->n_obs > SSMX_MAX_OBS)
+        return set_error(SSMX_EINVAL, "n_obs=%d outside 1..%d", d->n_obs,
+                         SSMX_MAX_OBS);
+    if (d->flags & ~SSMX_COEF_WAVE)
+        return set_error(SSMX_EINVAL,
+                         "flags=%d: the adjoint supports SSMX_COEF_WAVE only "
+                         "(non-rotating frame, integral readout)",
+                         d->flags);
+    if (d->first_out < 0)
+        return set_error(SSMX_EINVAL, "first_out=%d < 0", d->first_out);
+    if (d->partner != 0 && d->partner != 1)
+        return set_error(SSMX_EINVAL, "partner flag must be 0 or 1");
+    if (d->ckpt_every < 1)
+        return set_error(SSMX_EINVAL, "ckpt_every=%d < 1", d->ckpt_every);
+    if (d->n_theta < 0 || d->n_theta > SSMX_ADJ_MAX_THETA)
+        return set_error(SSMX_EINVAL, "n_theta=%d outside 0..%d", d->n_theta,
+                         SSMX_ADJ_MAX_THETA);
+    if (d->theta_gen < -1 || d->theta_gen >= d->n_gen)
+        return set_error(SSMX_EINVAL, "theta_gen=%d outside -1..%d",
+                         d->theta_gen, d->n_gen - 1);
+    if (d->n_theta > 0 && d->theta_gen < 0)
+        return set_error(SSMX_EINVAL, "derivative tables need theta_gen");
+    if (d->grad_gen < -1 || d->grad_gen >= d->n_gen)
+        return set_error(SSMX_EINVAL, "grad_gen=%d outside -1..%d",
+                         d->grad_gen, d->n_gen - 1);
+    if (d->grad_gen >= 0 && !(d->flags & SSMX_COEF_WAVE))
+        return set_error(SSMX_EINVAL,
+                         "segment gradients need waveform coefficients");
+    if (!(d->h > 0.0) || !(d->dt > 0.0))
+        return set_error(SSMX_EINVAL, "h and dt must be positive");
+    return SSMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssmx_adjoint_workspace_bytes(const ssmx_adjoint_desc *desc,
+                                 uint64_t *bytes)
+{
+    ssmx::error_buffer()[0] = '\0';
+    int rc = validate(desc);
+    if (rc)
+        return rc;
+    if (!bytes)
+        return set_error(SSMX_EINVAL, "null output pointer");
+    *bytes = (uint64_t)desc->n_inst * 2u * (uint64_t)desc->ckpt_every *
+             (uint64_t)desc->n_exp * kWave * sizeof(double);
+    return SSMX_OK;
+}
+
+int ssmx_adjoint(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
+                 const double *ell_val, const int32_t *ell_col_t,
+                 const double *ell_val_t, const double *dval_t,
+                 const double *cf_weights, const double *cf_nodes,
+                 const void *coef, const ssmx_segment *segments,
+                 const double *ckpt, const double *obs, const double *seed,
+                 double *workspace, double *grad_theta, double *grad_seg,
+                 double *lambda_out, void *stream)
+{
+    ssmx::error_buffer()[0] = '\0';
+    int rc = validate(desc);
+    if (rc)
+        return rc;
+    const bool wave_mode = (desc->flags & SSMX_COEF_WAVE) != 0;
+    if (!ell_col || !ell_val || !ell_col_t || !ell_val_t || !cf_weights ||
+        !cf_nodes || !coef || !ckpt || !obs || !seed || !workspace)
+        return set_error(SSMX_EINVAL, "null required pointer");
+    if (desc->n_theta > 0 && grad_theta && !dval_t)
+        return set_error(SSMX_EINVAL, "n_theta > 0 needs dval_t");
+    if (wave_mode && !segments)
+        return set_error(SSMX_EINVAL, "waveform mode needs segments");
+
+    AdjParams p;
+    p.d = *desc;
+    p.col_f = ell_col;
+    p.val_f = ell_val;
+    p.col_t = ell_col_t;
+    p.val_t = ell_val_t;
+    p.dval_t = dval_t;
+    p.w = cf_weights;
+    p.nodes = cf_nodes;
+    p.coef = wave_mode ? nullptr : static_cast<const double *>(coef);
+    p.waves = wave_mode ? static_cast<const ssmx_wave *>(coef) : nullptr;
+    p.segs = segments;
+    p.ckpt = ckpt;
+    p.obs = obs;
+    p.seed = seed;
+    p.work = workspace;
+    p.grad_theta = desc->n_theta > 0 ? grad_theta : nullptr;
+    p.grad_seg = desc->grad_gen >= 0 ? grad_seg : nullptr;
+    p.lambda_out = lambda_out;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    return dispatch(p, s);
+}
+
+}  // extern "C"

@@ -32,30 +32,20 @@
 // Generator values live in VGPRs when they fit, else in LDS shared by the
 // workgroup's wavefronts.
 
-#include <hip/hip_runtime.h>
-
-#include <cstdarg>
-#include <cstdint>
-#include <cstdio>
-#include <cstring>
-
-#include "../../include/ssmx.h"
+#include "ssmx_common.h"
 
 namespace {
 
-constexpr int kWave = 64;
+using ssmx::kWave;
+using ssmx::node_time;
+using ssmx::read_lane;
+using ssmx::set_error;
+using ssmx::swap_pair;
+using ssmx::wave_coef;
+using ssmx::wave_exchange;
+using ssmx::wave_sum;
+
 constexpr int kWavesPerBlock = 4;
-
-thread_local char g_error[512] = "";
-
-int set_error(int code, const char *fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_error, sizeof(g_error), fmt, ap);
-    va_end(ap);
-    return code;
-}
 
 // 1/q for the Taylor recursion, q = 0..40 (entry 0 unused).
 __constant__ double kInv[SSMX_MAX_TAYLOR + 1] = {
@@ -83,63 +73,6 @@ struct Params {
     double *out;
     double *state_out;
 };
-
-// Value of `v` held by lane ^ 1 (DPP quad_perm [1, 0, 3, 2]).
-__device__ __forceinline__ double swap_pair(double v)
-{
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF,
-                                            0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF,
-                                            0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Broadcast lane `src` (wave-uniform) of `v` to the whole wavefront.
-__device__ __forceinline__ double read_lane(double v, int src)
-{
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, kWave);
-    return v;
-}
-
-// Node time of global output interval i, fine division r, node c, in the
-// reference's order without contraction:
-//   ((t0 + (r dt) / F) + dt i) + (dt c) / F
-__device__ double node_time(double t0, double dt, int fine, int r, long i,
-                            double c)
-{
-#pragma clang fp contract(off)
-    const double offset = t0 + ((double)r * dt) / (double)fine;
-    const double start = offset + dt * (double)i;
-    return start + (dt * c) / (double)fine;
-}
-
-// Waveform coefficient g at time t (per lane); `cursor` only moves forward
-// because each lane's node times increase.
-__device__ double wave_coef(const ssmx_wave *wv, const ssmx_segment *segs,
-                            int g, double t, int &cursor)
-{
-#pragma clang fp contract(off)
-    double c = wv->base[g];
-    const int begin = wv->seg_begin[g];
-    const int count = wv->seg_count[g];
-    while (cursor < count && !(t < segs[begin + cursor].t_end))
-        ++cursor;
-    if (cursor < count) {
-        const ssmx_segment sg = segs[begin + cursor];
-        if (!(t < sg.t_start))
-            c += sg.amp * cos(sg.omega * t + sg.phase);
-    }
-    return c;
-}
 
 // Scalar factor of matrix m in exponential j at global step `step`:
 //   non-rotating (m = g):      h * sum_s w[j,s] c_g(t_s)
@@ -280,17 +213,13 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
         double g[SLOTS];
         g[0] = v;
         xbuf[lane] = v;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_exchange();
         if (SLOTS > 1)
             g[1] = use_partner ? swap_pair(v) : xbuf[col[1]];
 #pragma unroll
         for (int sl = 2; sl < SLOTS; ++sl)
             g[sl] = xbuf[col[sl]];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_exchange();
         double y0 = a[0] * g[0];
         double y1 = 0.0;
 #pragma unroll
@@ -303,12 +232,29 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
         return y0 + y1;
     };
 
+    // Checkpoints: with ckpt_every = K > 0, state_out receives the state
+    // before steps 0, K, 2K, ... instead of the final state.
+    const int ckpt_every = p.state_out ? d.ckpt_every : 0;
+    const long n_ckpt = ckpt_every > 0 ? (total + ckpt_every - 1) / ckpt_every
+                                       : 0;
+    int ckpt_count = 0;
+    long ckpt_index = 0;
+
     long step = 0;
     int b = 0;
     for (int i = 0; i < d.n_out; ++i) {
         for (int r = 0; r < d.fine; ++r, ++step, ++b) {
             if (b == B)
                 b = 0;
+            if (ckpt_every > 0) {
+                if (ckpt_count == 0 && lane < n)
+                    p.state_out[((size_t)inst * n_ckpt + ckpt_index) * n +
+                                lane] = x;
+                if (ckpt_count == 0)
+                    ++ckpt_index;
+                if (++ckpt_count == ckpt_every)
+                    ckpt_count = 0;
+            }
             if (b == 0) {
                 const long mine = step + my_b;
                 factor = (my_b < B && mine < total)
@@ -383,7 +329,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                 p.out[(size_t)inst * d.n_obs + 1] = v1 + off1;
         }
     }
-    if (p.state_out && lane < n)
+    if (p.state_out && ckpt_every == 0 && lane < n)
         p.state_out[(size_t)inst * n + lane] = x;
 }
 
@@ -462,6 +408,8 @@ int validate(const ssmx_propagate_desc *d)
         return set_error(SSMX_EINVAL, "first_out=%d < 0", d->first_out);
     if (d->partner != 0 && d->partner != 1)
         return set_error(SSMX_EINVAL, "partner flag must be 0 or 1");
+    if (d->ckpt_every < 0)
+        return set_error(SSMX_EINVAL, "ckpt_every=%d < 0", d->ckpt_every);
     return SSMX_OK;
 }
 
@@ -469,9 +417,9 @@ int validate(const ssmx_propagate_desc *d)
 
 extern "C" {
 
-int ssmx_version(void) { return 102; }
+int ssmx_version(void) { return 103; }
 
-const char *ssmx_last_error(void) { return g_error; }
+const char *ssmx_last_error(void) { return ssmx::error_buffer(); }
 
 int ssmx_device_count(void)
 {
@@ -491,7 +439,7 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
                    const double *obs_offset, double *out, double *state_out,
                    void *stream)
 {
-    g_error[0] = '\0';
+    ssmx::error_buffer()[0] = '\0';
     int rc = validate(desc);
     if (rc)
         return rc;

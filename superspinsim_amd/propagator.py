@@ -177,7 +177,7 @@ def taylor_plan(norm, tol=1e-17, max_taylor=_lib.MAX_TAYLOR):
 
 
 def taylor_plan_measured(exponents, tol=1e-17, safety=8.0,
-                         max_taylor=_lib.MAX_TAYLOR):
+                         max_taylor=_lib.MAX_TAYLOR, max_substeps=1024):
     """(m, s) from exact powers of representative exponents A.
 
     For each candidate substep count s and degree m the truncation error of
@@ -185,12 +185,12 @@ def taylor_plan_measured(exponents, tol=1e-17, safety=8.0,
         s * ||B^(m+1)||_1 / (m+1)! / (1 - ||B||_1 / (m+2)),  B = A/s,
     which is rigorous for the sampled A (||B^(m+1+i)|| <= ||B^(m+1)||
     ||B||^i); ``safety`` covers coefficients between the samples.  Returns
-    the cheapest (m * s) plan meeting ``tol``.
+    the cheapest (m * s) plan meeting ``tol`` with s <= ``max_substeps``.
     """
     exps = [np.asarray(a, dtype=_f64) for a in exponents]
     best = None
     s = 1
-    while s <= 1024:
+    while s <= max_substeps:
         scaled = [a / s for a in exps]
         norms = [float(np.abs(b).sum(axis=0).max()) for b in scaled]
         powers = [b.copy() for b in scaled]
@@ -389,10 +389,12 @@ class Propagator:
                     out.append(h * np.tensordot(cw, mats, axes=(0, 0)))
         return out
 
-    def plan(self, coef_samples, h, tol=1e-17):
+    def plan(self, coef_samples, h, tol=1e-17, max_substeps=1024,
+             max_taylor=_lib.MAX_TAYLOR):
         """Taylor plan from sampled exponents (see taylor_plan_measured)."""
         return taylor_plan_measured(self.sample_exponents(coef_samples, h),
-                                    tol=tol)
+                                    tol=tol, max_substeps=max_substeps,
+                                    max_taylor=max_taylor)
 
     def norm_bound(self, coef_abs_max, h):
         """Bound on ||A_j||_1 from per-node maxima |c[s, g]| (S, K)."""
@@ -411,13 +413,16 @@ class Propagator:
     def run(self, *, n_inst, fine, n_out, h, t0, dt, rho0, coef=None,
             coef_inst_stride=0, waves=None, segments=None, taylor=None,
             substeps=1, read="full", obs=None, obs_offset=None, out=None,
-            state_out=None, first_out=0, stream=None):
+            state_out=None, ckpt_every=0, first_out=0, stream=None):
         """Launch one propagation; returns the output tensor.
 
         ``rho0`` (n_inst, n) device tensor.  Coefficients: ``coef`` device
         table (..., n_out*fine*S*K) or ``waves``/``segments`` device buffers
         of ``ssmx_wave``/``ssmx_segment`` bytes.  ``read`` is "full",
-        "obs" or "integral".
+        "obs" or "integral".  ``state_out`` (device, float64) receives the
+        final state, or with ``ckpt_every = K > 0`` the state before every
+        K-th step, (n_inst, ceil(n_out * fine / K), n); both in the
+        propagator's internal basis (``perm`` order).
         """
         torch = self.torch
         flags = _lib.SSMX_ROTATING if self.rotating else 0
@@ -448,6 +453,7 @@ class Propagator:
             taylor=int(taylor), substeps=int(substeps), n_obs=max(n_obs, 1),
             flags=flags, first_out=int(first_out),
             partner=1 if self.ell.partner else 0,
+            ckpt_every=int(ckpt_every), reserved=0,
             coef_inst_stride=int(coef_inst_stride), h=float(h),
             t0=float(t0), dt=float(dt))
         obs_rows = None
@@ -456,8 +462,8 @@ class Propagator:
             rows[:, :self.n] = np.asarray(obs)[:, self.perm]
             obs_rows = device_tensor(rows)
         rho0 = rho0[:, self.d_perm].contiguous()
-        if state_out is not None:
-            raise ValueError("use run_state for the final state")
+        if ckpt_every and state_out is None:
+            raise ValueError("checkpoints need a state_out buffer")
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         lib = _lib.load()

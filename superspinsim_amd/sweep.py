@@ -39,11 +39,19 @@ class Waveform:
         if base is not None:
             self.base[:] = base
         self.segments = [[] for _ in range(n_gen)]
+        # per segment: (parameter index or -1, d amp / d parameter)
+        self.params = [[] for _ in range(n_gen)]
 
-    def add(self, g, t_start, t_end, amp, omega=0.0, phase=0.0):
-        """Add ``amp cos(omega t + phase)`` on [t_start, t_end) to c_g."""
+    def add(self, g, t_start, t_end, amp, omega=0.0, phase=0.0, param=-1,
+            dparam=1.0):
+        """Add ``amp cos(omega t + phase)`` on [t_start, t_end) to c_g.
+
+        ``param``/``dparam`` tag the segment for gradients: its ``amp``
+        depends on parameter ``param`` with derivative ``dparam``.
+        """
         self.segments[g].append((float(t_start), float(t_end), float(amp),
                                  float(omega), float(phase)))
+        self.params[g].append((int(param), float(dparam)))
         return self
 
     def tone(self, g, amp, frequency, t_start=-math.inf, t_end=math.inf,
@@ -92,16 +100,24 @@ WAVE_DTYPE = np.dtype([("base", _f64, _lib.MAX_GEN),
                        ("seg_count", np.int32, _lib.MAX_GEN)])
 
 
-def waveform_records(waves, n_gen):
-    """Host arrays matching ``ssmx_wave[]`` and ``ssmx_segment[]``."""
+def waveform_records(waves, n_gen, with_tags=False):
+    """Host arrays matching ``ssmx_wave[]`` and ``ssmx_segment[]``.
+
+    With ``with_tags`` also returns, per segment record, the parameter
+    index (-1 if untagged) and d amp / d parameter.  Every instance gets
+    its own records (the adjoint accumulates per record).
+    """
     recs = np.zeros(len(waves), dtype=WAVE_DTYPE)
     segs = []
+    tags = []
     for i, w in enumerate(waves):
         if w.n_gen != n_gen:
             raise ValueError("waveforms must cover the same generators")
         recs["base"][i, :n_gen] = w.base
         for g in range(n_gen):
-            ordered = sorted(w.segments[g])
+            order = sorted(range(len(w.segments[g])),
+                           key=lambda k: w.segments[g][k])
+            ordered = [w.segments[g][k] for k in order]
             for a, b in zip(ordered, ordered[1:]):
                 if b[0] < a[1]:
                     raise ValueError("overlapping segments on generator "
@@ -109,9 +125,16 @@ def waveform_records(waves, n_gen):
             recs["seg_begin"][i, g] = len(segs)
             recs["seg_count"][i, g] = len(ordered)
             segs.extend(ordered)
+            tags.extend(w.params[g][k] for k in order)
     if not segs:
         segs = [(0.0, 0.0, 0.0, 0.0, 0.0)]
-    return recs, np.array(segs, dtype=_f64).reshape(-1, 5)
+        tags = [(-1, 0.0)]
+    seg_arr = np.array(segs, dtype=_f64).reshape(-1, 5)
+    if not with_tags:
+        return recs, seg_arr
+    index = np.array([t[0] for t in tags], dtype=np.int64)
+    scale = np.array([t[1] for t in tags], dtype=_f64)
+    return recs, seg_arr, index, scale
 
 
 def pack_waveforms(waves, n_gen):
@@ -188,12 +211,14 @@ class BatchSimulator:
                 eig=eig)
         return self._prop_cache[key]
 
-    def plan(self, waves, time_step):
-        """(propagator, coefficient program, Taylor degree, substeps).
+    def program(self, waves):
+        """(active, const, program) of a batch of waveforms.
 
-        The coefficient program is a list of (generator index or None,
-        ...) describing the kernel's generators: None is the folded sum of
-        constant generators (coefficient 1), an int a varying generator.
+        ``active`` lists generators with a non-zero coefficient somewhere,
+        ``const`` maps generators whose coefficient is one constant for
+        every instance to that constant, and ``program`` is the kernel's
+        matrix list: None for the folded sum of constant generators
+        (coefficient 1), an int for a varying generator.
         """
         amax = np.max([w.abs_max() for w in waves], axis=0)
         active = [g for g in range(self.G) if amax[g] != 0] or [0]
@@ -202,10 +227,12 @@ class BatchSimulator:
             bases = {float(w.base[g]) for w in waves}
             if len(bases) == 1 and not any(w.segments[g] for w in waves):
                 const[g] = bases.pop()
-        prop = self._propagator(active, const, time_step)
         program = ([None] if const else []) + \
             [g for g in active if g not in const]
-        h = time_step / self.F
+        return active, const, program
+
+    def coefficient_box(self, waves, program):
+        """Node-coefficient samples (count, S, K) covering the batch."""
         lo = np.array([1.0 if g is None else
                        np.min([w.bounds()[0][g] for w in waves])
                        for g in program])
@@ -213,9 +240,16 @@ class BatchSimulator:
                        np.max([w.bounds()[1][g] for w in waves])
                        for g in program])
         S = self.nodes.size
-        samples = coefficient_samples(np.broadcast_to(lo, (S, lo.size)),
-                                      np.broadcast_to(hi, (S, hi.size)))
-        m, s = prop.plan(samples, h, tol=self.tolerance)
+        return coefficient_samples(np.broadcast_to(lo, (S, lo.size)),
+                                   np.broadcast_to(hi, (S, hi.size)))
+
+    def plan(self, waves, time_step):
+        """(propagator, coefficient program, Taylor degree, substeps)."""
+        active, const, program = self.program(waves)
+        prop = self._propagator(active, const, time_step)
+        h = time_step / self.F
+        m, s = prop.plan(self.coefficient_box(waves, program), h,
+                         tol=self.tolerance)
         return prop, program, m, s
 
     @staticmethod
@@ -228,6 +262,7 @@ class BatchSimulator:
             for k, g in enumerate(program):
                 if g is not None:
                     nw.segments[k] = list(w.segments[g])
+                    nw.params[k] = list(w.params[g])
             out.append(nw)
         return out
 

@@ -51,11 +51,17 @@ def test_struct_layout_matches_header(tmp_path):
 #include <stdio.h>
 #include "ssmx.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(ssmx_propagate_desc),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n",
+         sizeof(ssmx_propagate_desc),
          offsetof(ssmx_propagate_desc, coef_inst_stride),
          offsetof(ssmx_propagate_desc, h), offsetof(ssmx_propagate_desc, dt),
          sizeof(ssmx_segment), sizeof(ssmx_wave),
-         offsetof(ssmx_wave, seg_count));
+         offsetof(ssmx_wave, seg_count),
+         offsetof(ssmx_propagate_desc, ckpt_every),
+         sizeof(ssmx_adjoint_desc),
+         offsetof(ssmx_adjoint_desc, grad_gen),
+         offsetof(ssmx_adjoint_desc, coef_inst_stride),
+         offsetof(ssmx_adjoint_desc, dt));
   return 0;
 }
 ''')
@@ -65,9 +71,12 @@ int main(void) {
     got = [int(v) for v in subprocess.run(
         [str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     D = _lib.PropagateDesc
+    A = _lib.AdjointDesc
     mine = [ctypes.sizeof(D), D.coef_inst_stride.offset, D.h.offset,
             D.dt.offset, ctypes.sizeof(_lib.Segment),
-            ctypes.sizeof(_lib.Wave), _lib.Wave.seg_count.offset]
+            ctypes.sizeof(_lib.Wave), _lib.Wave.seg_count.offset,
+            D.ckpt_every.offset, ctypes.sizeof(A), A.grad_gen.offset,
+            A.coef_inst_stride.offset, A.dt.offset]
     assert got == mine
 
 
@@ -84,6 +93,7 @@ def _desc(**kw):
     dict(n=0), dict(n=65), dict(n_inst=0), dict(n_gen=9), dict(slots=0),
     dict(n_exp=7), dict(n_node=4), dict(taylor=0), dict(taylor=41),
     dict(substeps=3), dict(h=0.0), dict(first_out=-1), dict(partner=2),
+    dict(ckpt_every=-1),
     dict(flags=0, n_obs=3),
     dict(flags=_lib.SSMX_READ_FULL | _lib.SSMX_READ_INTEGRAL),
     dict(n_exp=6, n_node=3, n_gen=8, flags=_lib.SSMX_ROTATING |
@@ -104,6 +114,50 @@ def test_null_pointers_are_rejected(lib):
     assert b"null" in lib.ssmx_last_error()
     rc = lib.ssmx_propagate(None, *([None] * 13))
     assert rc == -1
+
+
+def _adesc(**kw):
+    base = dict(n=4, n_inst=1, n_gen=2, slots=2, slots_t=3, n_exp=2,
+                n_node=2, fine=1, n_out=1, taylor=8, taylor_adj=6, n_obs=1,
+                flags=_lib.SSMX_COEF_WAVE, first_out=0, partner=1,
+                ckpt_every=10, n_theta=1, theta_gen=0, grad_gen=1,
+                reserved=0, coef_inst_stride=0, h=1e-11, t0=0.0, dt=1e-11)
+    base.update(kw)
+    return _lib.AdjointDesc(**base)
+
+
+def test_adjoint_workspace_size(lib):
+    d = _adesc(n_inst=3, ckpt_every=7, n_exp=2)
+    out = ctypes.c_uint64(0)
+    assert lib.ssmx_adjoint_workspace_bytes(ctypes.byref(d),
+                                            ctypes.byref(out)) == 0
+    assert out.value == 3 * 2 * 7 * 2 * 64 * 8
+
+
+@pytest.mark.parametrize("bad", [
+    dict(n=65), dict(n_inst=0), dict(slots=33), dict(slots_t=0),
+    dict(taylor_adj=11), dict(taylor_adj=0), dict(ckpt_every=0),
+    dict(n_theta=17), dict(theta_gen=2), dict(theta_gen=-1),
+    dict(grad_gen=2), dict(flags=_lib.SSMX_ROTATING),
+    dict(flags=0, grad_gen=1), dict(n_obs=3), dict(n_out=0),
+    dict(n_node=4), dict(n_exp=7),
+])
+def test_invalid_adjoint_descriptor_is_rejected(lib, bad):
+    d = _adesc(**bad)
+    dummy = ctypes.c_void_p(8)
+    rc = lib.ssmx_adjoint(ctypes.byref(d), *([dummy] * 17))
+    assert rc == -1
+    assert lib.ssmx_last_error().decode()
+    out = ctypes.c_uint64(0)
+    assert lib.ssmx_adjoint_workspace_bytes(ctypes.byref(d),
+                                            ctypes.byref(out)) == -1
+
+
+def test_adjoint_null_pointers_are_rejected(lib):
+    d = _adesc()
+    rc = lib.ssmx_adjoint(ctypes.byref(d), *([None] * 17))
+    assert rc == -1
+    assert b"null" in lib.ssmx_last_error()
 
 
 def test_version(lib):
