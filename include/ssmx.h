@@ -185,7 +185,9 @@ typedef struct ssmx_adjoint_desc {
     int32_t fine;
     int32_t n_out;
     int32_t taylor;     /* degree of the forward (recomputation) */
-    int32_t taylor_adj; /* degree of the adjoint action, 1..10 */
+    int32_t taylor_adj; /* planned degree of the adjoint action, 1..10;
+                           the kernel evaluates degree 10 (a lower plan is
+                           raised, which only adds accuracy) */
     int32_t n_obs;      /* 1..2 */
     int32_t flags;      /* 0 or SSMX_COEF_WAVE */
     int32_t first_out;

@@ -34,10 +34,57 @@ import ctypes
 import numpy as np
 
 from superspinsim_amd import _lib
-from superspinsim_amd.propagator import EllStack, _ptr, _torch, device_tensor
+from superspinsim_amd.propagator import _ptr, _torch, device_tensor
 from superspinsim_amd.sweep import BatchSimulator, waveform_records
 
 _f64 = np.float64
+
+
+def shared_adjoint_ell(fwd_col, mats, dmats):
+    """One ELL pattern for the adjoint's forward and transposed products.
+
+    ``fwd_col`` is the forward kernel's column table (slots, 64), ``mats``
+    (M, n, n) and ``dmats`` (n_theta, n, n) are in the same (permuted)
+    basis.  Row r lists the forward columns of row r in the forward
+    kernel's slot order (so a product on this pattern adds the same terms
+    in the same order, plus exact zeros), then the other columns c with
+    A[c][r] or D[c][r] non-zero.  Returns (col, val_f, val_t, dval_t) with
+    val_f[m, s, r] = G_m[r][c], val_t[m, s, r] = G_m[c][r] and
+    dval_t[t, s, r] = D_t[c][r] for c = col[s, r].
+    """
+    M, n, _ = mats.shape
+    union = np.any(mats != 0, axis=0)
+    extra = union.T.copy()
+    if dmats.shape[0]:
+        extra |= np.any(dmats != 0, axis=0).T
+    rows = []
+    for r in range(n):
+        fwd = [int(c) for c in fwd_col[2:int(fwd_col[0, r]), r]]
+        special = {r, r ^ 1}
+        more = [int(c) for c in np.nonzero(extra[r])[0]
+                if c not in special and c not in fwd]
+        rows.append(fwd + more)
+    slots = 2 + max(len(c) for c in rows)
+    col = np.tile(np.arange(64, dtype=np.int32), (slots, 1))
+    for r in range(n):
+        col[1, r] = r ^ 1
+        for k, c in enumerate(rows[r]):
+            col[2 + k, r] = c
+    val_f = np.zeros((M, slots, 64))
+    val_t = np.zeros((M, slots, 64))
+    dval_t = np.zeros((dmats.shape[0], slots, 64))
+    for r in range(n):
+        for sl in range(slots):
+            c = int(col[sl, r])
+            if c >= n or (sl > 1 and sl >= 2 + len(rows[r])):
+                continue
+            val_f[:, sl, r] = mats[:, r, c]
+            val_t[:, sl, r] = mats[:, c, r]
+            if dmats.shape[0]:
+                dval_t[:, sl, r] = dmats[:, c, r]
+    for r in range(n):
+        col[0, r] = 2 + len(rows[r])
+    return col, val_f, val_t, dval_t
 
 
 class GradientProblem:
@@ -146,14 +193,11 @@ class GradientProblem:
         mats = prop._dense                                   # (M, n, n)
         mp = mats[:, perm][:, :, perm]
         dp = dconst[:, perm][:, :, perm]
-        stack = np.concatenate([np.transpose(mp, (0, 2, 1)),
-                                np.transpose(dp, (0, 2, 1))])
-        ellt = EllStack(stack, partner=True)
-        if ellt.slots > _lib.ADJ_MAX_SLOTS or \
-                prop.ell.slots > _lib.ADJ_MAX_SLOTS:
+        col, val_f, val_t, dval_t = shared_adjoint_ell(prop.ell.col, mp, dp)
+        if col.shape[0] > _lib.ADJ_MAX_SLOTS:
             raise NotImplementedError(
-                f"adjoint patterns up to {_lib.ADJ_MAX_SLOTS} slots "
-                f"(forward {prop.ell.slots}, transposed {ellt.slots})")
+                f"adjoint pattern of {col.shape[0]} slots > "
+                f"{_lib.ADJ_MAX_SLOTS}")
         M = mats.shape[0]
         torch = _torch()
         state = dict(
@@ -165,11 +209,12 @@ class GradientProblem:
             d_segs=device_tensor(seg_arr),
             tag_index=tag_index, tag_scale=tag_scale,
             n_records=seg_arr.shape[0],
-            d_col_t=device_tensor(ellt.col),
-            d_val_t=device_tensor(ellt.val[:M]),
-            d_dval_t=device_tensor(ellt.val[M:]) if dconst.shape[0] else
-            None,
-            slots_t=ellt.slots, n_inst=len(waves), dconst=dconst,
+            d_col=device_tensor(col),
+            # forward values are zero beyond the forward kernel's slots
+            d_val_f=device_tensor(val_f[:, :prop.ell.slots].copy()),
+            d_val_t=device_tensor(val_t),
+            d_dval_t=device_tensor(dval_t) if dconst.shape[0] else None,
+            slots_t=col.shape[0], n_inst=len(waves), dconst=dconst,
             kernel_waves=kwaves)
         return state
 
@@ -243,8 +288,8 @@ class GradientProblem:
                           device="cuda") if lambda_out else None
         stream = torch.cuda.current_stream().cuda_stream
         rc = lib.ssmx_adjoint(
-            ctypes.byref(desc), _ptr(prop.d_col), _ptr(prop.d_val),
-            _ptr(state["d_col_t"]), _ptr(state["d_val_t"]),
+            ctypes.byref(desc), _ptr(state["d_col"]), _ptr(state["d_val_f"]),
+            _ptr(state["d_col"]), _ptr(state["d_val_t"]),
             _ptr(state["d_dval_t"]), _ptr(prop.d_w), _ptr(prop.d_nodes),
             _ptr(state["d_waves"]), _ptr(state["d_segs"]),
             _ptr(state["ckpt"]), _ptr(d_obs), _ptr(seed), _ptr(work),

@@ -11,7 +11,9 @@
 // cotangent mu) accumulates the exponent gradient
 //   Abar = sum_{i + t < m_a} c_it u_i w_t^T,  u_i = (A^T)^i mu / i!,
 //   w_t = A^t y / t!,  c_it = i! t! / (i + t + 1)!,
-// which is the adjoint action of the block-triangular augmented exponential
+// (m_a = SSMX_ADJ_MAX_TAYLOR, a compile-time degree: the loops over the
+// adjoint and forward terms are fully unrolled and branch-free), which is
+// the adjoint action of the block-triangular augmented exponential
 // exp([[A, E], [0, A]]) whose upper block is the Frechet derivative of exp
 // at A in direction E: the powers of A (forward terms w_t) and of A^T
 // (adjoint terms u_i) are shared between the exponential and its
@@ -46,6 +48,11 @@ using ssmx::wave_sum;
 
 constexpr int kMaxAdj = SSMX_ADJ_MAX_TAYLOR;
 constexpr int kMaxSlots = 32;
+// Forward-term handoff ring between the two wavefronts of an instance.
+constexpr int kRing = 3;
+// Bound on the polls of one handoff wait; on overrun the kernel finishes
+// and reports NaN gradients instead of waiting on.
+constexpr long kMaxPolls = 10000000;
 
 // 1/q, q = 0..40 (entry 0 unused); the recompute uses the same table and
 // recursion as the propagator so that it reproduces the forward states.
@@ -186,11 +193,46 @@ __device__ __forceinline__ void assemble(double (&a)[SL], const double *lv,
     }
 }
 
-// Patterns up to 10 slots run two wavefronts per SIMD (256 VGPRs each);
-// wider ones one.
-template <int SF, int ST>
+// Two wavefronts per instance (one workgroup of 128 threads):
+//  wavefront 0 re-runs segment p - 1 from its checkpoint, storing every
+//    exponential's input in the workspace, and in the same loop evaluates
+//    the forward terms w_t = A^t y / t! of segment p's exponentials in
+//    reverse order (both are chains of products on the forward pattern, so
+//    one 16-byte exchange per term serves both); it reduces them to
+//    q_i = sum_t c_it w_t and hands q over through a double-buffered LDS
+//    slot;
+//  wavefront 1 runs the adjoint terms u_i of segment p in reverse order and
+//    accumulates the exponent gradient on the transposed pattern.
+// q travels through a ring of kRing LDS slots with two counters (produced,
+// consumed): wavefront 0 may run up to kRing exponentials ahead of
+// wavefront 1.  A workgroup barrier per segment ends a phase, after which
+// the recomputed inputs are read back; both wavefronts pass one barrier per
+// phase.
+// a[sl] = sum_m f_m g[m][sl] with the generator values g in registers; the
+// same order of operations as assemble().
+template <int SL, int MG>
+__device__ __forceinline__ void assemble_reg(double (&a)[SL],
+                                             const double (&g)[MG][SL],
+                                             double factor, int base)
+{
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl)
+        a[sl] = 0.0;
+#pragma unroll
+    for (int m = 0; m < MG; ++m) {
+        const double f = read_lane(factor, base + m);
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl)
+            a[sl] = fma(f, g[m][sl], a[sl]);
+    }
+}
+
+// MG = 0: generator values staged in LDS (any n_gen); MG = n_gen (1..3):
+// each wavefront keeps its generator values in registers, which removes
+// the LDS reads of every assembly.
+template <int SF, int ST, int MG>
 __global__ __attribute__((amdgpu_flat_work_group_size(2 * kWave, 2 * kWave),
-                          amdgpu_waves_per_eu(ST <= 10 ? 2 : 1))) void
+                          amdgpu_waves_per_eu(ST <= 16 ? 2 : 1))) void
 adjoint_kernel(AdjParams p)
 {
     constexpr int SFP = SF + (SF & 1);
@@ -199,31 +241,44 @@ adjoint_kernel(AdjParams p)
     extern __shared__ double lds[];
     const ssmx_adjoint_desc &d = p.d;
     const int M = d.n_gen;
-    // exchange buffers at LDS offset 0 (the buffer is an immediate offset
-    // of the gather instructions), then forward and transposed values
-    double *xb = lds;
-    double *lf = lds + 3 * kWave;
-    double *lt = lf + M * SFP * kWave;
-
-    for (int e = threadIdx.x; e < M * SFP * kWave; e += blockDim.x) {
-        const int half = e & 1;
-        const int row = (e >> 1) % kWave;
-        const int pair = (e / (2 * kWave)) % (SFP / 2);
-        const int m = e / (kWave * SFP);
-        const int slot = 2 * pair + half;
-        lf[e] = slot < d.slots
-                    ? p.val_f[((size_t)m * d.slots + slot) * kWave + row]
-                    : 0.0;
+    // LDS: exchange of wavefront 0 (64 x 16 B), exchange of wavefront 1
+    // (64 x 8 B), handoff [2][MA][64], staged forward / transposed values
+    double *x0 = lds;
+    double *x1 = lds + 2 * kWave;
+    double *hand = lds + 3 * kWave;
+    double *lf = hand + kRing * MA * kWave;
+    double *lt = lf + (MG == 0 ? M * SFP * kWave : 0);
+    // handoff counters after the staged values
+    int *produced =
+        reinterpret_cast<int *>(lt + (MG == 0 ? M * STP * kWave : 0));
+    int *consumed = produced + 1;
+    if (threadIdx.x == 0) {
+        *produced = 0;
+        *consumed = 0;
     }
-    for (int e = threadIdx.x; e < M * STP * kWave; e += blockDim.x) {
-        const int half = e & 1;
-        const int row = (e >> 1) % kWave;
-        const int pair = (e / (2 * kWave)) % (STP / 2);
-        const int m = e / (kWave * STP);
-        const int slot = 2 * pair + half;
-        lt[e] = slot < d.slots_t
-                    ? p.val_t[((size_t)m * d.slots_t + slot) * kWave + row]
-                    : 0.0;
+
+    if constexpr (MG == 0) {
+        for (int e = threadIdx.x; e < M * SFP * kWave; e += blockDim.x) {
+            const int half = e & 1;
+            const int row = (e >> 1) % kWave;
+            const int pair = (e / (2 * kWave)) % (SFP / 2);
+            const int m = e / (kWave * SFP);
+            const int slot = 2 * pair + half;
+            lf[e] = slot < d.slots
+                        ? p.val_f[((size_t)m * d.slots + slot) * kWave + row]
+                        : 0.0;
+        }
+        for (int e = threadIdx.x; e < M * STP * kWave; e += blockDim.x) {
+            const int half = e & 1;
+            const int row = (e >> 1) % kWave;
+            const int pair = (e / (2 * kWave)) % (STP / 2);
+            const int m = e / (kWave * STP);
+            const int slot = 2 * pair + half;
+            lt[e] = slot < d.slots_t
+                        ? p.val_t[((size_t)m * d.slots_t + slot) * kWave +
+                                  row]
+                        : 0.0;
+        }
     }
     __syncthreads();
 
@@ -241,82 +296,256 @@ adjoint_kernel(AdjParams p)
     const double *coef_i =
         wave_mode ? nullptr : p.coef + (size_t)inst * d.coef_inst_stride;
     const ssmx_wave *wv = wave_mode ? p.waves + inst : nullptr;
-    const int grad_gen = p.grad_seg ? d.grad_gen : -1;
-    const int theta_gen = d.theta_gen;
-    const int ma = d.taylor_adj;
+    const size_t seg_stride = (size_t)K * J * kWave;
 
-    int cf[SF];
-    cf[0] = lane;
-#pragma unroll
-    for (int sl = 1; sl < SF; ++sl)
-        cf[sl] = sl < d.slots ? p.col_f[sl * kWave + lane] : lane;
-    const double inv_lane = kInvA[lane <= SSMX_MAX_TAYLOR ? lane : 0];
-
-    // Both wavefronts pass the same number of workgroup barriers (P + 1 in
-    // the loops below, plus the staging barrier above); the branch on the
-    // wavefront's role is uniform within each wavefront.
     if (role == 0) {
-        for (int q = 0; q <= P; ++q) {
-            // ---------------- recompute segment P - 1 - q ----------------
-            if (q < P) {
-                const int seg = P - 1 - q;
-                const long k0 = (long)seg * K;
-                const long k1 = k0 + K < total ? k0 + K : total;
-                double *wb = p.work + ((size_t)inst * 2 + (seg & 1)) *
-                                          (size_t)K * J * kWave;
-                double *xr = xb;
-                double x = lane < n ? p.ckpt[((size_t)inst * P + seg) * n +
-                                             lane]
+        // ------------- recomputation + forward terms -------------
+        int cf[SF];
+        cf[0] = lane;
+#pragma unroll
+        for (int sl = 1; sl < SF; ++sl)
+            cf[sl] = sl < d.slots ? p.col_f[sl * kWave + lane] : lane;
+        const double inv_lane = kInvA[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+        const int JM = J * M;
+        const int B = kWave / JM;          // steps per coefficient batch
+        const int my_b = lane / JM;
+        const int my_j = (lane % JM) / M;
+        const int my_m = lane % M;
+        const int m_fwd = d.taylor;
+        int cur_r = 0, cur_v = 0;
+        double2 *xp = reinterpret_cast<double2 *>(x0);
+        double gf[MG > 0 ? MG : 1][SF];
+        if constexpr (MG > 0) {
+#pragma unroll
+            for (int m = 0; m < MG; ++m)
+#pragma unroll
+                for (int sl = 0; sl < SF; ++sl)
+                    gf[m][sl] = sl < d.slots
+                                    ? p.val_f[((size_t)m * d.slots + sl) *
+                                                  kWave +
+                                              lane]
                                     : 0.0;
-                const int JM = J * M;
-                const int B = kWave / JM;
-                const int my_b = lane / JM;
-                const int my_j = (lane % JM) / M;
-                const int my_m = lane % M;
-                int cursor = 0;
-                double factor = 0.0;
-                int b = 0;
-                for (long k = k0; k < k1; ++k, ++b) {
-                    if (b == B)
-                        b = 0;
-                    if (b == 0) {
-                        const long mine = k + my_b;
-                        factor = (my_b < B && mine < k1)
-                                     ? step_factor<false>(p, coef_i, wv, mine,
-                                                          my_j, my_m, cursor)
-                                     : 0.0;
-                    }
-                    for (int j = J - 1; j >= 0; --j) {
-                        wb[((size_t)(k - k0) * J + j) * kWave + lane] = x;
-                        double a[SF];
-                        assemble<SF>(a, lf, factor, (b * J + j) * M, M, lane);
-                        double term = x;
-                        double acc = x;
-                        for (int t = 1; t <= d.taylor; ++t) {
-                            term = ell_product<SF>(a, cf, term, xr, lane,
-                                                   use_partner) *
-                                   read_lane(inv_lane, t);
-                            acc += term;
+        }
+        long handed = 0;                   // exponentials handed over
+        int slot = 0;                      // ring slot of the next one
+        bool broken = false;
+
+        for (int q = 0; q <= P; ++q) {
+            const bool r_seg = q < P;      // recompute segment P - 1 - q
+            const bool v_seg = q >= 1;     // forward terms of segment P - q
+            const int rseg = r_seg ? P - 1 - q : 0;
+            const int vseg = v_seg ? P - q : 0;
+            const long rk0 = (long)rseg * K;
+            const long rk1 = r_seg ? (rk0 + K < total ? rk0 + K : total) : rk0;
+            const long vk0 = (long)vseg * K;
+            const long vk1 = v_seg ? (vk0 + K < total ? vk0 + K : total) : vk0;
+            const long ER = (rk1 - rk0) * J;
+            const long EV = (vk1 - vk0) * J;
+            double *wr = p.work + ((size_t)inst * 2 + (rseg & 1)) * seg_stride;
+            const double *wvin =
+                p.work + ((size_t)inst * 2 + (vseg & 1)) * seg_stride;
+
+            double x = (r_seg && lane < n)
+                           ? p.ckpt[((size_t)inst * P + rseg) * n + lane]
+                           : 0.0;
+            long rk = rk0;                 // recompute: step, exponential,
+            int rj = J - 1;                // position in the batch
+            int rb = 0;
+            cur_r = 0;                     // forward-only cursor: restart
+            double rfac = 0.0;
+            long vk = vk1 - 1;             // forward terms, reverse order
+            int vj = 0;
+            long vkb = vk + B;             // batch top (forces a load)
+            double vfac = 0.0;
+            long pk = vk;                  // prefetched input (clamped)
+            int pj = vj;
+            double ypre = EV > 0 ? wvin[((size_t)(pk - vk0) * J + pj) *
+                                            kWave + lane]
+                                 : 0.0;
+            const long EMAX = ER > EV ? ER : EV;
+            for (long e = 0; e < EMAX; ++e) {
+                const bool r_on = e < ER;
+                const bool v_on = e < EV;
+                double aR[SF], aV[SF];
+#pragma unroll
+                for (int sl = 0; sl < SF; ++sl) {
+                    aR[sl] = 0.0;
+                    aV[sl] = 0.0;
+                }
+                if (r_on) {
+                    if (rj == J - 1) {
+                        if (rb == B)
+                            rb = 0;
+                        if (rb == 0) {
+                            const long mine = rk + my_b;
+                            rfac = (my_b < B && mine < rk1)
+                                       ? step_factor<false>(p, coef_i, wv,
+                                                            mine, my_j, my_m,
+                                                            cur_r)
+                                       : 0.0;
                         }
-                        x = acc;
+                    }
+                    wr[((size_t)(rk - rk0) * J + rj) * kWave + lane] = x;
+                    if constexpr (MG > 0)
+                        assemble_reg<SF, MG>(aR, gf, rfac, (rb * J + rj) * M);
+                    else
+                        assemble<SF>(aR, lf, rfac, (rb * J + rj) * M, M, lane);
+                }
+                double W[MA];
+                W[0] = ypre;
+                if (v_on) {
+                    if (vk <= vkb - B) {
+                        vkb = vk;
+                        const long st = vk - my_b;
+                        vfac = (my_b < B && st >= vk0)
+                                   ? step_factor<true>(p, coef_i, wv, st,
+                                                       my_j, my_m, cur_v)
+                                   : 0.0;
+                    }
+                    const int vbase = ((int)(vkb - vk) * J + vj) * M;
+                    if constexpr (MG > 0)
+                        assemble_reg<SF, MG>(aV, gf, vfac, vbase);
+                    else
+                        assemble<SF>(aV, lf, vfac, vbase, M, lane);
+                    // prefetch the next exponential's input (clamped to
+                    // the segment's last one)
+                    if (++pj == J) {
+                        pj = 0;
+                        --pk;
+                    }
+                    if (pk < vk0) {
+                        pk = vk0;
+                        pj = J - 1;
+                    }
+                    ypre = wvin[((size_t)(pk - vk0) * J + pj) * kWave + lane];
+                }
+                // fused chains: recompute term t and forward term t
+                double term = x;
+                double acc = x;
+#pragma unroll
+                for (int t = 1; t < MA; ++t) {
+                    xp[lane] = make_double2(term, W[t - 1]);
+                    ssmx::wave_exchange();
+                    double gR[SF], gV[SF];
+                    gR[0] = term;
+                    gV[0] = W[t - 1];
+                    if (use_partner) {
+                        gR[1] = swap_pair(term);
+                        gV[1] = swap_pair(W[t - 1]);
+                    } else {
+                        const double2 g = xp[cf[1]];
+                        gR[1] = g.x;
+                        gV[1] = g.y;
+                    }
+#pragma unroll
+                    for (int sl = 2; sl < SF; ++sl) {
+                        const double2 g = xp[cf[sl]];
+                        gR[sl] = g.x;
+                        gV[sl] = g.y;
+                    }
+                    ssmx::wave_exchange();
+                    // the propagator's product: same terms, same order
+                    double y0 = aR[0] * gR[0];
+                    double y1 = 0.0;
+                    double v0 = aV[0] * gV[0];
+                    double v1 = 0.0;
+#pragma unroll
+                    for (int sl = 1; sl < SF; ++sl) {
+                        if (sl & 1) {
+                            y1 = fma(aR[sl], gR[sl], y1);
+                            v1 = fma(aV[sl], gV[sl], v1);
+                        } else {
+                            y0 = fma(aR[sl], gR[sl], y0);
+                            v0 = fma(aV[sl], gV[sl], v0);
+                        }
+                    }
+                    {
+                        // the propagator's roundings (no contraction)
+#pragma clang fp contract(off)
+                        term = (y0 + y1) * (1.0 / t);
+                        if (t <= m_fwd)
+                            acc += term;
+                    }
+                    W[t] = (v0 + v1) * (1.0 / t);
+                }
+                for (int t = MA; t <= m_fwd; ++t) {
+#pragma clang fp contract(off)
+                    term = ell_product<SF>(aR, cf, term, x0, lane,
+                                           use_partner) *
+                           read_lane(inv_lane, t);
+                    acc += term;
+                }
+                if (r_on) {
+                    x = acc;
+                    if (rj == 0) {
+                        rj = J - 1;
+                        ++rk;
+                        ++rb;
+                    } else {
+                        --rj;
+                    }
+                }
+                if (v_on) {
+                    // q_i = sum_t c_it w_t, c_i0 = 1 / (i + 1),
+                    // c_it = c_i,t-1 * t / (i + t + 1)
+                    double Q[MA];
+#pragma unroll
+                    for (int i = 0; i < MA; ++i) {
+                        double c = 1.0 / (i + 1);
+                        double qv = c * W[0];
+#pragma unroll
+                        for (int t = 1; t + i < MA; ++t) {
+                            c = c * t / (i + t + 1);
+                            qv = fma(c, W[t], qv);
+                        }
+                        Q[i] = qv;
+                    }
+                    // wait for a free slot, fill it, publish it
+                    for (long poll = 0;
+                         !broken &&
+                         handed - __hip_atomic_load(
+                                      consumed, __ATOMIC_ACQUIRE,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP) >=
+                             kRing;
+                         ++poll) {
+                        if (poll == kMaxPolls)
+                            broken = true;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    double *hb = hand + slot * MA * kWave;
+#pragma unroll
+                    for (int i = 0; i < MA; ++i)
+                        hb[i * kWave + lane] = Q[i];
+                    if (++slot == kRing)
+                        slot = 0;
+                    ++handed;
+                    if (lane == 0)
+                        __hip_atomic_store(produced, (int)handed,
+                                           __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (++vj == J) {
+                        vj = 0;
+                        --vk;
                     }
                 }
             }
-            __syncthreads();
+            __syncthreads();               // end of phase
         }
         return;
     }
 
-    // ---- adjoint wavefront state (persists over segments) ----
+    // ------------- adjoint terms -------------
     int ct[ST];
     ct[0] = lane;
 #pragma unroll
     for (int sl = 1; sl < ST; ++sl)
         ct[sl] = sl < d.slots_t ? p.col_t[sl * kWave + lane] : lane;
+    const int grad_gen = p.grad_seg ? d.grad_gen : -1;
+    const int theta_gen = d.theta_gen;
     double seedrow = 0.0;
     for (int o = 0; o < d.n_obs; ++o)
-        seedrow = fma(p.seed[(size_t)inst * d.n_obs + o], p.obs[o * kWave + lane],
-                      seedrow);
+        seedrow = fma(p.seed[(size_t)inst * d.n_obs + o],
+                      p.obs[o * kWave + lane], seedrow);
     double lam = (0.5 * d.h) * seedrow;
     double Sd[ST];
 #pragma unroll
@@ -325,7 +554,17 @@ adjoint_kernel(AdjParams p)
     int cur_rec = -1;
     int cursor_a = 0;
     const double *lt_grad = lt + (grad_gen >= 0 ? grad_gen : 0) * STP * kWave;
-
+    double gt[MG > 0 ? MG : 1][ST];
+    if constexpr (MG > 0) {
+#pragma unroll
+        for (int m = 0; m < MG; ++m)
+#pragma unroll
+            for (int sl = 0; sl < ST; ++sl)
+                gt[m][sl] =
+                    sl < d.slots_t
+                        ? p.val_t[((size_t)m * d.slots_t + sl) * kWave + lane]
+                        : 0.0;
+    }
     // Amplitude gradient of the current segment record: per-lane partial
     // sums of <Abar, G_grad_gen>, reduced over the wavefront only when the
     // record changes.
@@ -338,27 +577,24 @@ adjoint_kernel(AdjParams p)
             amp_acc = 0.0;
         }
     };
+    long taken = 0;                     // exponentials taken over
+    int slot = 0;                       // ring slot of the next one
+    bool broken = false;
+    const int JM = J * M;
+    const int WB = JM + S;              // batch lanes per step
+    const int BA = kWave / WB;          // steps per batch
+    const int my_b = lane / WB;
+    const int my_r = lane % WB;
 
     for (int q = 0; q <= P; ++q) {
-        if (q > 0) {
-            // ---------------- adjoint of segment P - q ----------------
+        if (q >= 1) {
             const int seg = P - q;
             const long k0 = (long)seg * K;
             const long k1 = k0 + K < total ? k0 + K : total;
-            const double *wb = p.work + ((size_t)inst * 2 + (seg & 1)) *
-                                            (size_t)K * J * kWave;
-            double *xu = xb + kWave;
-            double *xv = xb + 2 * kWave;
             const long E = (k1 - k0) * J;
-            const int JM = J * M;
-            const int WB = JM + S;          // batch lanes per step
-            const int BA = kWave / WB;      // steps per batch
-            const int my_b = lane / WB;
-            const int my_r = lane % WB;
-            long kb = 0;                    // batch holds kb, kb - 1, ...
+            long kb = 0;                // batch holds kb, kb - 1, ...
             double fac = 0.0, damp_l = 0.0;
             int rec_l = -1;
-
             auto load_batch = [&](long top) __attribute__((always_inline)) {
                 kb = top;
                 const long st = top - my_b;
@@ -375,90 +611,46 @@ adjoint_kernel(AdjParams p)
                     }
                 }
             };
-            auto y_at = [&](long idx) __attribute__((always_inline)) -> double {
-                const long k = k1 - 1 - idx / J;
-                const int j = (int)(idx % J);
-                return wb[((size_t)(k - k0) * J + j) * kWave + lane];
-            };
-
-            // current exponential: step ck, exponential cj
             long ck = k1 - 1;
             int cj = 0;
             load_batch(ck);
-            double a_f[SF], a_t[ST];
-            int base = (int)(kb - ck) * WB;
-            assemble<SF>(a_f, lf, fac, base + cj * M, M, lane);
-            assemble<ST>(a_t, lt, fac, base + cj * M, M, lane);
-            double f_dep = theta_gen >= 0
-                               ? read_lane(fac, base + cj * M + theta_gen)
-                               : 0.0;
-            int rec_s[SSMX_MAX_NODE];
-            double dw_s[SSMX_MAX_NODE];
-            auto extract_amp = [&](int bse, int j) __attribute__((always_inline)) {
-#pragma unroll
-                for (int s = 0; s < SSMX_MAX_NODE; ++s) {
-                    rec_s[s] = -1;
-                    dw_s[s] = 0.0;
-                    if (grad_gen >= 0 && s < S) {
-                        rec_s[s] = __builtin_amdgcn_readlane(rec_l,
-                                                             bse + JM + s);
-                        dw_s[s] = (d.h * p.w[j * S + s]) *
-                                  read_lane(damp_l, bse + JM + s);
-                    }
+            for (long e = 0; e < E; ++e) {
+                if (ck <= kb - BA)
+                    load_batch(ck);
+                const int base = (int)(kb - ck) * WB;
+                double a_t[ST];
+                if constexpr (MG > 0)
+                    assemble_reg<ST, MG>(a_t, gt, fac, base + cj * M);
+                else
+                    assemble<ST>(a_t, lt, fac, base + cj * M, M, lane);
+                const double f_dep =
+                    theta_gen >= 0 ? read_lane(fac, base + cj * M + theta_gen)
+                                   : 0.0;
+                // wait until q of this exponential is published
+                for (long poll = 0;
+                     !broken &&
+                     __hip_atomic_load(produced, __ATOMIC_ACQUIRE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) <=
+                         taken;
+                     ++poll) {
+                    if (poll == kMaxPolls)
+                        broken = true;
+                    __builtin_amdgcn_s_sleep(1);
                 }
-            };
-            extract_amp(base, cj);
-
-            // forward terms of the first exponential
-            double W[MA];
-#pragma unroll
-            for (int t = 0; t < MA; ++t)
-                W[t] = 0.0;
-            W[0] = y_at(0);
-#pragma unroll
-            for (int t = 1; t < MA; ++t) {
-                if (t < ma)
-                    W[t] = ell_product<SF>(a_f, cf, W[t - 1], xv, lane,
-                                           use_partner) *
-                           (1.0 / t);
-            }
-            double ypre = E > 1 ? y_at(1) : 0.0;
-
-            for (long idx = 0; idx < E; ++idx) {
-                // q_i = sum_t c_it w_t with c_i0 = 1 / (i + 1) and
-                // c_it = c_i,t-1 * t / (i + t + 1); the loops are unrolled,
-                // so every c_it is a compile-time constant.  Terms with
-                // t >= m_a vanish (W[t] = 0 there).
+                const double *hb = hand + slot * MA * kWave;
+                if (++slot == kRing)
+                    slot = 0;
                 double Q[MA];
 #pragma unroll
-                for (int i = 0; i < MA; ++i) {
-                    double c = 1.0 / (i + 1);
-                    double qv = c * W[0];
-#pragma unroll
-                    for (int t = 1; t + i < MA; ++t) {
-                        c = c * t / (i + t + 1);
-                        qv = fma(c, W[t], qv);
-                    }
-                    Q[i] = qv;
-                }
-                const bool has_next = idx + 1 < E;
-                long nk = ck;
-                int nj = cj + 1;
-                if (nj == J) {
-                    nj = 0;
-                    --nk;
-                }
-                if (has_next) {
-                    if (nk <= kb - BA)
-                        load_batch(nk);
-                    const int nbase = (int)(kb - nk) * WB;
-                    assemble<SF>(a_f, lf, fac, nbase + nj * M, M, lane);
-#pragma unroll
-                    for (int t = 0; t < MA; ++t)
-                        W[t] = 0.0;
-                    W[0] = ypre;
-                    ypre = idx + 2 < E ? y_at(idx + 2) : 0.0;
-                }
+                for (int i = 0; i < MA; ++i)
+                    Q[i] = hb[i * kWave + lane];
+                // the slot is free once its values are in registers
+                ssmx::wave_exchange();
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+                ++taken;
+                if (lane == 0)
+                    __hip_atomic_store(consumed, (int)taken, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
 
                 double u = lam;
                 double lam_new = lam;
@@ -468,31 +660,15 @@ adjoint_kernel(AdjParams p)
                     T[sl] = 0.0;
 #pragma unroll
                 for (int i = 0; i < MA; ++i) {
-                    if (i >= ma)
-                        continue;
-                    const bool fwd = has_next && i + 1 < ma;
-                    xu[lane] = u;
-                    if (fwd)
-                        xv[lane] = W[i];
-                    wave_exchange();
+                    x1[lane] = u;
+                    ssmx::wave_exchange();
                     double gu[ST];
                     gu[0] = u;
-                    gu[1] = use_partner ? swap_pair(u) : xu[ct[1]];
+                    gu[1] = use_partner ? swap_pair(u) : x1[ct[1]];
 #pragma unroll
                     for (int sl = 2; sl < ST; ++sl)
-                        gu[sl] = xu[ct[sl]];
-                    double gv[SF];
-                    if (fwd) {
-                        gv[0] = W[i];
-                        gv[1] = use_partner ? swap_pair(W[i]) : xv[cf[1]];
-#pragma unroll
-                        for (int sl = 2; sl < SF; ++sl)
-                            gv[sl] = xv[cf[sl]];
-                    }
-                    wave_exchange();
-#pragma unroll
-                    for (int sl = 0; sl < ST; ++sl)
-                        T[sl] = fma(gu[sl], Q[i], T[sl]);
+                        gu[sl] = x1[ct[sl]];
+                    ssmx::wave_exchange();
                     double u0 = a_t[0] * gu[0];
                     double u1 = 0.0;
 #pragma unroll
@@ -504,18 +680,13 @@ adjoint_kernel(AdjParams p)
                     }
                     u = (u0 + u1) * (1.0 / (i + 1));
                     lam_new += u;
-                    if (fwd) {
-                        double v0 = a_f[0] * gv[0];
-                        double v1 = 0.0;
+                    // The empty asm keeps each accumulation here: T is only
+                    // read in conditional blocks below, and sinking the
+                    // chain there would keep every term's gathers alive.
 #pragma unroll
-                        for (int sl = 1; sl < SF; ++sl) {
-                            if (sl & 1)
-                                v1 = fma(a_f[sl], gv[sl], v1);
-                            else
-                                v0 = fma(a_f[sl], gv[sl], v0);
-                        }
-                        W[i + 1 < MA ? i + 1 : MA - 1] =
-                            (v0 + v1) * (1.0 / (i + 1));
+                    for (int sl = 0; sl < ST; ++sl) {
+                        T[sl] = fma(gu[sl], Q[i], T[sl]);
+                        asm volatile("" : "+v"(T[sl]));
                     }
                 }
                 lam = lam_new;
@@ -529,87 +700,106 @@ adjoint_kernel(AdjParams p)
                 if (grad_gen >= 0) {
                     // <Abar, G_grad_gen> restricted to this lane's column
                     double tg = 0.0;
-                    const double2 *g2 =
-                        reinterpret_cast<const double2 *>(lt_grad) + lane;
+                    if constexpr (MG > 0) {
 #pragma unroll
-                    for (int pr = 0; pr < STP / 2; ++pr) {
-                        const double2 gg = g2[pr * kWave];
-                        tg = fma(T[2 * pr], gg.x, tg);
-                        if (2 * pr + 1 < ST)
-                            tg = fma(T[2 * pr + 1], gg.y, tg);
+                        for (int m = 0; m < MG; ++m)
+                            if (m == grad_gen) {
+#pragma unroll
+                                for (int sl = 0; sl < ST; ++sl)
+                                    tg = fma(T[sl], gt[m][sl], tg);
+                            }
+                    } else {
+                        const double2 *g2 =
+                            reinterpret_cast<const double2 *>(lt_grad) + lane;
+#pragma unroll
+                        for (int pr = 0; pr < STP / 2; ++pr) {
+                            const double2 gg = g2[pr * kWave];
+                            tg = fma(T[2 * pr], gg.x, tg);
+                            if (2 * pr + 1 < ST)
+                                tg = fma(T[2 * pr + 1], gg.y, tg);
+                        }
                     }
 #pragma unroll
                     for (int s = 0; s < SSMX_MAX_NODE; ++s) {
-                        if (s < S && rec_s[s] >= 0) {
-                            if (rec_s[s] != cur_rec) {
-                                flush();
-                                cur_rec = rec_s[s];
+                        if (s < S) {
+                            const int rec =
+                                __builtin_amdgcn_readlane(rec_l, base + JM + s);
+                            if (rec >= 0) {
+                                const double dw =
+                                    (d.h * p.w[cj * S + s]) *
+                                    read_lane(damp_l, base + JM + s);
+                                if (rec != cur_rec) {
+                                    flush();
+                                    cur_rec = rec;
+                                }
+                                amp_acc = fma(dw, tg, amp_acc);
                             }
-                            amp_acc = fma(dw_s[s], tg, amp_acc);
                         }
                     }
                 }
                 if (cj == J - 1)
                     lam = fma(ck == 0 ? 0.5 * d.h : d.h, seedrow, lam);
-
-                if (has_next) {
-                    const int nbase = (int)(kb - nk) * WB;
-                    assemble<ST>(a_t, lt, fac, nbase + nj * M, M, lane);
-                    f_dep = theta_gen >= 0
-                                ? read_lane(fac, nbase + nj * M + theta_gen)
-                                : 0.0;
-                    extract_amp(nbase, nj);
-                    ck = nk;
-                    cj = nj;
+                if (++cj == J) {
+                    cj = 0;
+                    --ck;
                 }
             }
         }
-        __syncthreads();
+        __syncthreads();                   // end of phase
     }
 
-    {
-        flush();
-        if (p.grad_theta) {
-            for (int t = 0; t < d.n_theta; ++t) {
-                double v = 0.0;
+    flush();
+    if (broken) {
+        // a handoff wait overran its bound: report NaN instead of values
+        const double nan = __builtin_nan("");
 #pragma unroll
-                for (int sl = 0; sl < ST; ++sl)
-                    if (sl < d.slots_t)
-                        v = fma(Sd[sl],
-                                p.dval_t[((size_t)t * d.slots_t + sl) * kWave +
-                                         lane],
-                                v);
-                v = wave_sum(v);
-                if (lane == 0)
-                    p.grad_theta[(size_t)inst * d.n_theta + t] = v;
-            }
-        }
-        if (p.lambda_out && lane < n)
-            p.lambda_out[(size_t)inst * n + lane] = lam;
+        for (int sl = 0; sl < ST; ++sl)
+            Sd[sl] = nan;
+        lam = nan;
     }
+    if (p.grad_theta) {
+        for (int t = 0; t < d.n_theta; ++t) {
+            double v = 0.0;
+#pragma unroll
+            for (int sl = 0; sl < ST; ++sl)
+                if (sl < d.slots_t)
+                    v = fma(Sd[sl],
+                            p.dval_t[((size_t)t * d.slots_t + sl) * kWave +
+                                     lane],
+                            v);
+            v = wave_sum(v);
+            if (lane == 0)
+                p.grad_theta[(size_t)inst * d.n_theta + t] = v;
+        }
+    }
+    if (p.lambda_out && lane < n)
+        p.lambda_out[(size_t)inst * n + lane] = lam;
 }
 
-template <int SF, int ST>
+template <int SF, int ST, int MG = 0>
 int launch(const AdjParams &p, hipStream_t stream)
 {
     constexpr int SFP = SF + (SF & 1);
     constexpr int STP = ST + (ST & 1);
     const int M = p.d.n_gen;
-    const size_t lds =
-        ((size_t)M * (SFP + STP) * kWave + 3 * kWave) * sizeof(double);
+    // exchange buffers, handoff ring, staged values, two int counters
+    const size_t staged = MG == 0 ? (size_t)M * (SFP + STP) * kWave : 0;
+    const size_t lds = (staged + 3 * kWave + (size_t)kRing * kMaxAdj * kWave +
+                        1) *
+                       sizeof(double);
     if (lds > 160 * 1024)
         return set_error(SSMX_EINVAL,
                          "adjoint tables need %zu bytes of LDS (> 160 KiB)",
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&adjoint_kernel<SF, ST>),
+            reinterpret_cast<const void *>(&adjoint_kernel<SF, ST, MG>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
-    hipLaunchKernelGGL((adjoint_kernel<SF, ST>), dim3(p.d.n_inst),
+    hipLaunchKernelGGL((adjoint_kernel<SF, ST, MG>), dim3(p.d.n_inst),
                        dim3(2 * kWave), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -632,6 +822,19 @@ int dispatch(const AdjParams &p, hipStream_t s)
             pick = i;
             break;
         }
+    // two or three matrices: generator values in registers
+    const int M = p.d.n_gen;
+    if (M == 2 || M == 3) {
+        const bool two = M == 2;
+        switch (pick) {
+        case 0: return two ? launch<4, 4, 2>(p, s) : launch<4, 4, 3>(p, s);
+        case 1: return two ? launch<4, 7, 2>(p, s) : launch<4, 7, 3>(p, s);
+        case 2: return two ? launch<7, 7, 2>(p, s) : launch<7, 7, 3>(p, s);
+        case 3: return two ? launch<4, 10, 2>(p, s) : launch<4, 10, 3>(p, s);
+        case 4: return two ? launch<7, 10, 2>(p, s) : launch<7, 10, 3>(p, s);
+        default: break;
+        }
+    }
     switch (pick) {
     case 0: return launch<4, 4>(p, s);
     case 1: return launch<4, 7>(p, s);

@@ -291,6 +291,9 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                     double term = x;
                     double acc = x;
                     for (int t = 1; t <= d.taylor; ++t) {
+                        // no contraction: the adjoint's recomputation
+                        // repeats these two roundings
+#pragma clang fp contract(off)
                         term = product(a, term) * read_lane(inv_lane, t);
                         acc += term;
                     }
