@@ -1,5 +1,6 @@
 """Benchmark: propagator steps/s of the 7-level NV ODMR sweep, FP64, and
-the adjoint/forward cost ratio of a full parameter gradient.
+the adjoint/forward cost ratio of a full parameter gradient, and the
+throughput of a batch of uneven pulsed protocols.
 
 One bench step is one whole sweep: every instance (sweep point) advanced
 through all of its propagator steps (h = 20 ps, CF4:2) in one launch of the
@@ -12,6 +13,10 @@ The gradient leg runs the 1024-point version of the sweep with a
 then the adjoint (recomputation + reverse sweep) for the squared error
 against targets simulated at parameters perturbed by up to 2 %; the ratio
 is (forward with checkpoints + backward) / plain forward, wall time.
+
+The protocol leg runs 16384 pulsed Ramsey / Hahn-echo protocols of 58k-742k
+steps in one persistent work-queue launch (autotuned shape, replayed from a
+HIP graph) and reports propagator steps/s over the whole batch.
 
 Prints one JSON line (rank 0, N = 1 GPU).
 """
@@ -61,6 +66,12 @@ def parse():
                    help="run only the gradient leg (profiling)")
     p.add_argument("--adjoint-instances", type=int, default=1024)
     p.add_argument("--adjoint-steps", type=int, default=2)
+    p.add_argument("--no-protocols", action="store_true",
+                   help="skip the uneven-protocol (work-queue) leg")
+    p.add_argument("--protocols-only", action="store_true")
+    p.add_argument("--protocol-count", type=int, default=16384)
+    p.add_argument("--protocol-steps", type=int, default=1,
+                   help="timed graph replays of the protocol batch")
     return p.parse_args()
 
 
@@ -187,6 +198,56 @@ def adjoint_leg(args):
     }
 
 
+def protocol_leg(args):
+    """16384 uneven pulsed protocols in one persistent launch (graph)."""
+    import numpy as np
+    import torch
+    from superspinsim_amd import autotune, configs
+    wl = configs.pulsed7(args.protocol_count)
+    batch = wl.batch().prepare(wl.waves, wl.rho0, wl.time_step, wl.steps,
+                               wl.read_from, wl.obs)
+    total = batch.total_steps()
+    print(f"protocol workload {wl.name}: {batch.n_inst} protocols, "
+          f"{total:.3e} steps ({wl.steps.min()}-{wl.steps.max()} each)",
+          file=sys.stderr, flush=True)
+    choice = autotune.tune(batch, trial_steps=20000, repeats=1)
+    print(f"autotuner: {choice['waves_per_block']} wavefronts/workgroup, "
+          f"m={choice['taylor']}, s={choice['substeps']}", file=sys.stderr,
+          flush=True)
+    batch.capture()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for _ in range(args.protocol_steps):
+        batch.graph.replay()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.protocol_steps
+    kernel_s = ev[0].elapsed_time(ev[1]) / 1e3 / args.protocol_steps
+    out = batch.out.cpu().numpy()
+    flops = batch.step_flops() * total
+    return {
+        "workload": wl.name,
+        "protocols": batch.n_inst,
+        "steps_total": total,
+        "steps_min": int(wl.steps.min()),
+        "steps_max": int(wl.steps.max()),
+        "readout": "integrated fluorescence over the final 1 us window",
+        "launch": "persistent work-queue kernel, longest first, "
+                  "hipGraph replay",
+        "autotuner": {k: choice[k] for k in ("waves_per_block", "taylor",
+                                             "substeps", "cached")},
+        "wavefronts": batch.n_waves,
+        "s_per_batch": wall,
+        "kernel_s": kernel_s,
+        "steps_per_s": total / wall,
+        "achieved_tflops": flops / kernel_s / 1e12,
+        "finite": bool(np.all(np.isfinite(out))),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -209,6 +270,9 @@ def main():
 
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(args)}))
+        return
+    if args.protocols_only:
+        print(json.dumps({"protocols": protocol_leg(args)}))
         return
 
     wl = workload(args.config, args.instances)
@@ -321,6 +385,8 @@ def main():
     if not args.no_adjoint and args.config == "odmr7":
         result["adjoint"] = adjoint_leg(args)
         result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]
+    if not args.no_protocols and args.config == "odmr7":
+        result["protocols"] = protocol_leg(args)
     if not args.no_cpu:
         outputs = args.cpu_outputs or (128 if wl.n > 20 else 999)
         result["cpu_baseline"] = cpu_baseline(wl, outputs)

@@ -145,6 +145,44 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
                    void *stream);
 
 /*
+ * Persistent work-queue propagation of protocols with uneven lengths.
+ *
+ * Instance i runs steps[i] propagator steps (node times as ssmx_propagate
+ * with first_out = 0; n_out is unused) and writes the trapezoid integral of
+ * its readout rows over steps read_from[i] .. steps[i]:
+ *   out[i][o] = sum_{k = read_from[i]}^{steps[i]-1} (h/2) obs_o.(x_k + x_k+1)
+ * desc->flags must hold SSMX_COEF_WAVE | SSMX_READ_INTEGRAL (and may hold
+ * SSMX_ROTATING).  The grid holds n_waves wavefronts in blocks of
+ * waves_per_block (1..8); each wavefront takes queue entries q = 0, 1, ...
+ * from *counter with an atomic increment and runs instance order[q] to the
+ * end, so order the entries longest first.  *counter must be 0 at launch
+ * (clear it with a kernel on the same stream, not a memset node, when the
+ * launch is captured in a graph); it ends at n_inst + n_waves.
+ *
+ * waves      ssmx_wave [n_inst]; segments as ssmx_propagate
+ * rho0       double [n_inst * n]
+ * obs        double [n_obs * 64]
+ * order      int32 [n_inst], a permutation of 0 .. n_inst - 1
+ * steps      int64 [n_inst]; read_from int64 [n_inst] (0 <= read_from)
+ * counter    int32 [1] device
+ * out        double [n_inst * n_obs]
+ *
+ * No counterpart in the reference (one simulate call per protocol); each
+ * instance runs the path ssmx_propagate replaces
+ * (generate_simulator.py:1945-2116).
+ */
+int ssmx_propagate_queue(const ssmx_propagate_desc *desc,
+                         const int32_t *ell_col, const double *ell_val,
+                         const double *cf_weights, const double *cf_nodes,
+                         const ssmx_wave *waves,
+                         const ssmx_segment *segments, const double *rho0,
+                         const double *eig, const double *obs,
+                         const int32_t *order, const int64_t *steps,
+                         const int64_t *read_from, int32_t *counter,
+                         int32_t n_waves, int32_t waves_per_block,
+                         double *out, void *stream);
+
+/*
  * Adjoint (reverse-mode gradient) of the trapezoid-integrated readout
  *   F_o = sum_k w_k obs_o . x_k,  w_0 = w_T = h / 2, other w_k = h,
  * over the same propagation as ssmx_propagate with SSMX_READ_INTEGRAL

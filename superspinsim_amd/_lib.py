@@ -79,8 +79,9 @@ class Wave(ctypes.Structure):
 
 
 #: every symbol include/ssmx.h declares
-EXPORTS = ("ssmx_propagate", "ssmx_adjoint", "ssmx_adjoint_workspace_bytes",
-           "ssmx_last_error", "ssmx_version", "ssmx_device_count")
+EXPORTS = ("ssmx_propagate", "ssmx_propagate_queue", "ssmx_adjoint",
+           "ssmx_adjoint_workspace_bytes", "ssmx_last_error", "ssmx_version",
+           "ssmx_device_count")
 
 _lib = None
 
@@ -102,6 +103,10 @@ def load():
     vp = ctypes.c_void_p
     lib.ssmx_propagate.argtypes = [ctypes.POINTER(PropagateDesc)] + [vp] * 13
     lib.ssmx_propagate.restype = ctypes.c_int
+    lib.ssmx_propagate_queue.argtypes = (
+        [ctypes.POINTER(PropagateDesc)] + [vp] * 13 +
+        [ctypes.c_int32, ctypes.c_int32, vp, vp])
+    lib.ssmx_propagate_queue.restype = ctypes.c_int
     lib.ssmx_adjoint.argtypes = [ctypes.POINTER(AdjointDesc)] + [vp] * 17
     lib.ssmx_adjoint.restype = ctypes.c_int
     lib.ssmx_adjoint_workspace_bytes.argtypes = [

@@ -22,6 +22,16 @@
   rate, ground T1 and T2*) and the 200 amplitudes; the readout is the
   integrated fluorescence, checkpoints every 1000 steps.
 
+* ``pulsed7`` - 16384 pulsed protocols on the ``odmr7`` model (n = 49),
+  seed 2: pulse length t_p ~ U(10 ns, 1 us) (a pi/2 pulse; the echo's pi
+  pulse lasts 2 t_p), free precession tau ~ U(0.1, 10 us), Hahn echo
+  (pi/2 - tau/2 - pi - tau/2 - pi/2) or Ramsey (pi/2 - tau - pi/2) with
+  probability 1/2, microwave detuning ~ U(-20, 20) MHz from the
+  ms = 0 <-> -1 line, Rabi frequency 3 MHz, start in ms = 0 of the ground
+  triplet, then a 1 us optical readout (pumping rate 5/65 of the radiative
+  rate) whose fluorescence is integrated.  h = 20 ps; 56k-700k steps per
+  protocol.
+
 Drive amplitude convention (the reference docs' ``examples.rst``):
 ``B1 = sqrt(2) * Omega / 28 GHz/T`` for a Rabi frequency Omega (Hz).
 """
@@ -236,3 +246,77 @@ def odmr7_gradient(count=1024, segments=200, seed=3, time_end=2e-6,
         "odmr7-gradient-n19"
     return GradientWorkload(name, problem, nv7_theta0(), amplitudes, freqs,
                             seed)
+
+
+# ---------------------------------------------------------------------------
+# Uneven pulsed protocols
+# ---------------------------------------------------------------------------
+
+class ProtocolWorkload:
+    """Frozen protocol batch: waveforms, step counts and readout windows."""
+
+    def __init__(self, name, generators, waves, rho0, obs, time_step, steps,
+                 read_from, params):
+        self.name = name
+        self.generators = generators
+        self.waves = waves
+        self.rho0 = rho0
+        self.obs = obs
+        self.time_step = time_step
+        self.steps = steps
+        self.read_from = read_from
+        self.params = params
+
+    def batch(self, tolerance=1e-16):
+        from superspinsim_amd.protocols import ProtocolBatch
+        return ProtocolBatch(self.generators, 2, tolerance=tolerance)
+
+
+def pulsed7(count=16384, seed=2, time_step=20e-12, readout=1e-6,
+            rabi_hz=3e6, use_incoherent=False, max_count=None,
+            t_p_range=(10e-9, 1e-6), tau_range=(0.1e-6, 10e-6)):
+    """Pulsed Ramsey / Hahn-echo protocols on the 7-level model.
+
+    ``t_p_range``/``tau_range``/``readout`` shrink the protocols for tests.
+    """
+    base = odmr7(1, use_incoherent=use_incoherent)
+    rng = np.random.default_rng(seed)
+    t_p = rng.uniform(t_p_range[0], t_p_range[1], count)
+    tau = rng.uniform(tau_range[0], tau_range[1], count)
+    echo = rng.random(count) < 0.5
+    detuning = rng.uniform(-20e6, 20e6, count)
+    if max_count is not None:
+        t_p, tau, echo, detuning = (t_p[:max_count], tau[:max_count],
+                                    echo[:max_count], detuning[:max_count])
+        count = len(t_p)
+    ground = params.nv.room.ground
+    line = 2.87e9 - ground.g_longitudinal * params.fundamental.bohr_gyro * \
+        5e-3
+    b1 = drive_amplitude(rabi_hz)
+    waves, steps, read_from = [], [], []
+    for k in range(count):
+        omega = math.tau * (line + detuning[k])
+        w = Waveform(5, base=[1.0, 0.0, 0.0, 0.0, 0.0])
+        t = 0.0
+        if echo[k]:
+            pulses = [(t_p[k], 0.5 * tau[k]), (2 * t_p[k], 0.5 * tau[k]),
+                      (t_p[k], 0.0)]
+        else:
+            pulses = [(t_p[k], tau[k]), (t_p[k], 0.0)]
+        for length, wait in pulses:
+            w.add(1, t, t + length, b1, omega, 0.0)
+            t += length + wait
+        t_read = t
+        t_end = t_read + readout
+        w.add(4, t_read, t_end, base.excitation)
+        waves.append(w)
+        steps.append(int(math.ceil(t_end / time_step)))
+        read_from.append(int(round(t_read / time_step)))
+    rho = np.zeros((base.hilbert, base.hilbert))
+    rho[1, 1] = 1.0                       # ground ms = 0
+    rho0 = base._flat(rho)
+    return ProtocolWorkload(
+        "pulsed7-nv-n49" if not use_incoherent else "pulsed7-nv-n19",
+        base.generators, waves, rho0, base.obs, time_step,
+        np.array(steps, dtype=np.int64), np.array(read_from, dtype=np.int64),
+        dict(t_p=t_p, tau=tau, echo=echo, detuning=detuning, line=line))

@@ -1,0 +1,386 @@
+// ssmx_queue.hip - persistent work-queue propagator for batches of uneven
+// protocols (gfx950).
+//
+// Every instance has its own number of propagator steps and its own readout
+// window; the integrated readout F_o = sum over the window of the trapezoid
+// (h/2)(obs_o . x_k + obs_o . x_{k+1}) is written per instance.  The grid is
+// sized to the wavefronts the device holds at once; each wavefront takes the
+// next instance from an atomic counter (in the caller's order, normally
+// longest first) until the queue is empty, so the long protocols start first
+// and the short ones fill the tail.
+//
+// The per-step arithmetic is the propagator's (ssmx_propagate.hip): same
+// node times, coefficient evaluation, assembly, Taylor action, product order
+// and rotating eigen-step, so an instance gives the same states here as in a
+// launch of its own.
+
+#include "ssmx_common.h"
+
+namespace {
+
+using ssmx::kWave;
+using ssmx::node_time;
+using ssmx::read_lane;
+using ssmx::set_error;
+using ssmx::swap_pair;
+using ssmx::wave_coef;
+using ssmx::wave_exchange;
+using ssmx::wave_sum;
+
+constexpr int kMaxWavesPerBlock = 8;
+
+__constant__ double kInvQ[SSMX_MAX_TAYLOR + 1] = {
+    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
+    1.0 / 6,    1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
+    1.0 / 12,   1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
+    1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
+    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
+    1.0 / 30,   1.0 / 31,   1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35,
+    1.0 / 36,   1.0 / 37,   1.0 / 38, 1.0 / 39, 1.0 / 40};
+
+struct QParams {
+    ssmx_propagate_desc d;
+    const int32_t *ell_col;
+    const double *ell_val;
+    const double *w;
+    const double *nodes;
+    const ssmx_wave *waves;
+    const ssmx_segment *segs;
+    const double *rho0;
+    const double *eig;
+    const double *obs;
+    const int32_t *order;
+    const int64_t *steps;
+    const int64_t *read_from;
+    int32_t *counter;
+    double *out;
+};
+
+// Scalar factor of matrix m in exponential j at step `step` of the
+// instance whose waveform is `wv` (the propagator's step_factor, wave mode).
+__device__ double step_factor_q(const QParams &p, const ssmx_wave *wv,
+                                long step, int j, int m, int &cursor)
+{
+    const ssmx_propagate_desc &d = p.d;
+    const int S = d.n_node, K = d.n_gen;
+    const long i_loc = step / d.fine;
+    const int r = (int)(step - i_loc * d.fine);
+    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
+    if (rotating) {
+        const int s = m / K, g = m % K;
+        const double c = wave_coef(
+            wv, p.segs, g, node_time(d.t0, d.dt, d.fine, r, i_loc, p.nodes[s]),
+            cursor);
+        return c * (d.h * p.w[j * S + s]);
+    }
+    double cw = 0.0;
+    for (int s = 0; s < S; ++s) {
+        const double c = wave_coef(
+            wv, p.segs, m, node_time(d.t0, d.dt, d.fine, r, i_loc, p.nodes[s]),
+            cursor);
+        cw = fma(p.w[j * S + s], c, cw);
+    }
+    return d.h * cw;
+}
+
+template <int SLOTS>
+__global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
+    QParams p)
+{
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    extern __shared__ double lds[];
+    const ssmx_propagate_desc &d = p.d;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
+    const int M = (rotating ? d.n_node : 1) * d.n_gen;
+    {
+        const int nval = M * SLOTP * kWave;
+        for (int e = threadIdx.x; e < nval; e += blockDim.x) {
+            const int half = e & 1;
+            const int row = (e >> 1) % kWave;
+            const int pair = (e / (2 * kWave)) % (SLOTP / 2);
+            const int m = e / (kWave * SLOTP);
+            const int slot = 2 * pair + half;
+            lds[e] = slot < d.slots
+                         ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
+                                     row]
+                         : 0.0;
+        }
+        __syncthreads();
+    }
+    double *xbuf = lds + M * SLOTP * kWave + (threadIdx.x / kWave) * kWave;
+
+    const int n = d.n;
+    const int J = d.n_exp;
+    const bool use_partner = d.partner != 0;
+    int col[SLOTS];
+    col[0] = lane;
+#pragma unroll
+    for (int s = 1; s < SLOTS; ++s)
+        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
+    const double inv_lane = kInvQ[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+    double eig_own = 0.0, eig_part = 0.0;
+    if (rotating) {
+        eig_own = p.eig[lane];
+        eig_part = p.eig[kWave + lane];
+    }
+    const double obs0 = p.obs[lane];
+    const double obs1 = d.n_obs > 1 ? p.obs[kWave + lane] : 0.0;
+    const int JM = J * M;
+    const int B = kWave / JM;
+    const int my_b = lane / JM;
+    const int my_j = (lane % JM) / M;
+    const int my_m = lane % M;
+    const double inv_sub = 1.0 / (double)d.substeps;
+    const double half_h = 0.5 * d.h;
+
+    auto product = [&](const double (&a)[SLOTS], double v) -> double {
+        double g[SLOTS];
+        g[0] = v;
+        xbuf[lane] = v;
+        wave_exchange();
+        if (SLOTS > 1)
+            g[1] = use_partner ? swap_pair(v) : xbuf[col[1]];
+#pragma unroll
+        for (int sl = 2; sl < SLOTS; ++sl)
+            g[sl] = xbuf[col[sl]];
+        wave_exchange();
+        double y0 = a[0] * g[0];
+        double y1 = 0.0;
+#pragma unroll
+        for (int sl = 1; sl < SLOTS; ++sl) {
+            if (sl & 1)
+                y1 = fma(a[sl], g[sl], y1);
+            else
+                y0 = fma(a[sl], g[sl], y0);
+        }
+        return y0 + y1;
+    };
+
+    // Every pass takes one queue entry; the counter only grows, so every
+    // wavefront leaves the loop once the n_inst entries are handed out.
+    for (;;) {
+        int q = 0;
+        if (lane == 0)
+            q = atomicAdd(p.counter, 1);
+        q = __builtin_amdgcn_readfirstlane(q);
+        if (q >= d.n_inst)
+            break;
+        const int inst = p.order[q];
+        const long total = p.steps[inst];
+        const long k_from = p.read_from[inst];
+        const ssmx_wave *wv = p.waves + inst;
+        double x = lane < n ? p.rho0[(size_t)inst * n + lane] : 0.0;
+        double xint = 0.0;
+        int cursor = 0;
+        double factor = 0.0;
+        int b = 0;
+        for (long step = 0; step < total; ++step, ++b) {
+            if (b == B)
+                b = 0;
+            if (b == 0) {
+                const long mine = step + my_b;
+                factor = (my_b < B && mine < total)
+                             ? step_factor_q(p, wv, mine, my_j, my_m, cursor)
+                             : 0.0;
+            }
+            const double x_prev = x;
+            for (int j = J - 1; j >= 0; --j) {
+                double a[SLOTS];
+#pragma unroll
+                for (int sl = 0; sl < SLOTS; ++sl)
+                    a[sl] = 0.0;
+                const int base = (b * J + j) * M;
+                for (int m = 0; m < M; ++m) {
+                    const double f = read_lane(factor, base + m);
+                    const double2 *v =
+                        reinterpret_cast<const double2 *>(lds +
+                                                          m * SLOTP * kWave) +
+                        lane;
+#pragma unroll
+                    for (int pr = 0; pr < SLOTP / 2; ++pr) {
+                        const double2 vv = v[pr * kWave];
+                        a[2 * pr] = fma(f, vv.x, a[2 * pr]);
+                        if (2 * pr + 1 < SLOTS)
+                            a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
+                    }
+                }
+                if (d.substeps > 1) {
+#pragma unroll
+                    for (int sl = 0; sl < SLOTS; ++sl)
+                        a[sl] *= inv_sub;
+                }
+                for (int sub = 0; sub < d.substeps; ++sub) {
+                    double term = x;
+                    double acc = x;
+                    for (int t = 1; t <= d.taylor; ++t) {
+#pragma clang fp contract(off)
+                        term = product(a, term) * read_lane(inv_lane, t);
+                        acc += term;
+                    }
+                    x = acc;
+                }
+            }
+            if (rotating) {
+                const double xp = swap_pair(x);
+                x = x + fma(eig_own, x, eig_part * xp);
+            }
+            if (step >= k_from)
+                xint = fma(half_h, x_prev + x, xint);
+        }
+        const double v0 = wave_sum(obs0 * xint);
+        if (lane == 0)
+            p.out[(size_t)inst * d.n_obs] = v0;
+        if (d.n_obs > 1) {
+            const double v1 = wave_sum(obs1 * xint);
+            if (lane == 0)
+                p.out[(size_t)inst * d.n_obs + 1] = v1;
+        }
+    }
+}
+
+template <int SLOTS>
+int launch(const QParams &p, int n_waves, int waves_per_block,
+           hipStream_t stream)
+{
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
+    const size_t lds = ((size_t)M * SLOTP * kWave +
+                        (size_t)waves_per_block * kWave) *
+                       sizeof(double);
+    if (lds > 160 * 1024)
+        return set_error(SSMX_EINVAL,
+                         "generator values need %zu bytes of LDS (> 160 KiB)",
+                         lds);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&queue_kernel<SLOTS>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
+                             hipGetErrorString(e));
+    }
+    const int blocks = (n_waves + waves_per_block - 1) / waves_per_block;
+    hipLaunchKernelGGL((queue_kernel<SLOTS>), dim3(blocks),
+                       dim3(kWave * waves_per_block), lds, stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(SSMX_ELAUNCH, "queue launch: %s",
+                         hipGetErrorString(e));
+    return SSMX_OK;
+}
+
+int validate(const ssmx_propagate_desc *d, int n_waves, int waves_per_block)
+{
+    if (!d)
+        return set_error(SSMX_EINVAL, "null descriptor");
+    if (d->n < 1 || d->n > SSMX_MAX_N)
+        return set_error(SSMX_EINVAL, "n=%d outside 1..%d", d->n, SSMX_MAX_N);
+    if (d->n_inst < 1)
+        return set_error(SSMX_EINVAL, "n_inst=%d < 1", d->n_inst);
+    if (d->n_gen < 1 || d->n_gen > SSMX_MAX_GEN)
+        return set_error(SSMX_EINVAL, "n_gen=%d outside 1..%d", d->n_gen,
+                         SSMX_MAX_GEN);
+    if (d->slots < 1 || d->slots > SSMX_MAX_N)
+        return set_error(SSMX_EINVAL, "slots=%d outside 1..64", d->slots);
+    if (d->n_exp < 1 || d->n_exp > SSMX_MAX_EXP)
+        return set_error(SSMX_EINVAL, "n_exp=%d outside 1..%d", d->n_exp,
+                         SSMX_MAX_EXP);
+    if (d->n_node < 1 || d->n_node > SSMX_MAX_NODE)
+        return set_error(SSMX_EINVAL, "n_node=%d outside 1..%d", d->n_node,
+                         SSMX_MAX_NODE);
+    const int M = ((d->flags & SSMX_ROTATING) ? d->n_node : 1) * d->n_gen;
+    if (d->n_exp * M > kWave)
+        return set_error(SSMX_EINVAL,
+                         "n_exp * matrices = %d exceeds one batch of 64",
+                         d->n_exp * M);
+    if (d->fine < 1)
+        return set_error(SSMX_EINVAL, "fine=%d < 1", d->fine);
+    if (d->taylor < 1 || d->taylor > SSMX_MAX_TAYLOR)
+        return set_error(SSMX_EINVAL, "taylor=%d outside 1..%d", d->taylor,
+                         SSMX_MAX_TAYLOR);
+    if (d->substeps < 1 || (d->substeps & (d->substeps - 1)) != 0 ||
+        d->substeps > 1024)
+        return set_error(SSMX_EINVAL,
+                         "substeps=%d is not a power of two <= 1024",
+                         d->substeps);
+    if (d->n_obs < 1 || d->n_obs > 2)
+        return set_error(SSMX_EINVAL, "n_obs=%d outside 1..2", d->n_obs);
+    if ((d->flags & ~(SSMX_ROTATING | SSMX_COEF_WAVE | SSMX_READ_INTEGRAL)) ||
+        !(d->flags & SSMX_COEF_WAVE) || !(d->flags & SSMX_READ_INTEGRAL))
+        return set_error(SSMX_EINVAL,
+                         "queue mode needs SSMX_COEF_WAVE | "
+                         "SSMX_READ_INTEGRAL (optionally SSMX_ROTATING)");
+    if (!(d->h > 0.0) || !(d->dt > 0.0))
+        return set_error(SSMX_EINVAL, "h and dt must be positive");
+    if (d->partner != 0 && d->partner != 1)
+        return set_error(SSMX_EINVAL, "partner flag must be 0 or 1");
+    if (n_waves < 1)
+        return set_error(SSMX_EINVAL, "n_waves=%d < 1", n_waves);
+    if (waves_per_block < 1 || waves_per_block > kMaxWavesPerBlock)
+        return set_error(SSMX_EINVAL, "waves_per_block=%d outside 1..%d",
+                         waves_per_block, kMaxWavesPerBlock);
+    return SSMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssmx_propagate_queue(const ssmx_propagate_desc *desc,
+                         const int32_t *ell_col, const double *ell_val,
+                         const double *cf_weights, const double *cf_nodes,
+                         const ssmx_wave *waves,
+                         const ssmx_segment *segments, const double *rho0,
+                         const double *eig, const double *obs,
+                         const int32_t *order, const int64_t *steps,
+                         const int64_t *read_from, int32_t *counter,
+                         int32_t n_waves, int32_t waves_per_block,
+                         double *out, void *stream)
+{
+    ssmx::error_buffer()[0] = '\0';
+    int rc = validate(desc, n_waves, waves_per_block);
+    if (rc)
+        return rc;
+    if (!ell_col || !ell_val || !cf_weights || !cf_nodes || !waves ||
+        !segments || !rho0 || !obs || !order || !steps || !read_from ||
+        !counter || !out)
+        return set_error(SSMX_EINVAL, "null required pointer");
+    if ((desc->flags & SSMX_ROTATING) && !eig)
+        return set_error(SSMX_EINVAL, "rotating frame needs eig factors");
+    QParams p;
+    p.d = *desc;
+    p.ell_col = ell_col;
+    p.ell_val = ell_val;
+    p.w = cf_weights;
+    p.nodes = cf_nodes;
+    p.waves = waves;
+    p.segs = segments;
+    p.rho0 = rho0;
+    p.eig = eig;
+    p.obs = obs;
+    p.order = order;
+    p.steps = steps;
+    p.read_from = read_from;
+    p.counter = counter;
+    p.out = out;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (desc->slots) {
+    case 1: case 2: return launch<2>(p, n_waves, waves_per_block, s);
+    case 3: return launch<3>(p, n_waves, waves_per_block, s);
+    case 4: return launch<4>(p, n_waves, waves_per_block, s);
+    case 5: return launch<5>(p, n_waves, waves_per_block, s);
+    case 6: return launch<6>(p, n_waves, waves_per_block, s);
+    case 7: return launch<7>(p, n_waves, waves_per_block, s);
+    case 8: return launch<8>(p, n_waves, waves_per_block, s);
+    case 9: case 10: return launch<10>(p, n_waves, waves_per_block, s);
+    case 11: case 12: return launch<12>(p, n_waves, waves_per_block, s);
+    default: break;
+    }
+    if (desc->slots <= 16) return launch<16>(p, n_waves, waves_per_block, s);
+    if (desc->slots <= 24) return launch<24>(p, n_waves, waves_per_block, s);
+    if (desc->slots <= 32) return launch<32>(p, n_waves, waves_per_block, s);
+    return launch<64>(p, n_waves, waves_per_block, s);
+}
+
+}  // extern "C"

@@ -168,3 +168,27 @@ def test_overlapping_segments_rejected():
     w.add(0, 1.0, 3.0, 1.0)
     with pytest.raises(ValueError):
         waveform_records([w], 1)
+
+
+def test_pulsed_protocol_workload():
+    from superspinsim_amd import configs
+    wl = configs.pulsed7(count=512)
+    assert len(wl.waves) == 512
+    assert np.all(wl.read_from < wl.steps)
+    # readout window is 1 us = 50,000 steps of 20 ps (rounding of the ends)
+    assert np.all(np.abs((wl.steps - wl.read_from) - 50000) <= 1)
+    assert wl.steps.min() >= 0.1e-6 / 20e-12
+    assert wl.steps.max() <= (4e-6 + 10e-6 + 1e-6) / 20e-12 + 1
+    for w, ro in zip(wl.waves[:5], wl.read_from[:5]):
+        # laser off before the readout window, on inside it
+        assert w(ro * 20e-12 - 1e-9)[4] == 0.0
+        assert w(ro * 20e-12 + 1e-9)[4] > 0.0
+
+
+def test_trapezoid_window():
+    from superspinsim_amd.protocols import trapezoid_window
+    s = np.arange(11, dtype=float)[None] ** 2
+    got = trapezoid_window(s, [2], [7], 0.5)
+    want = 0.5 * 0.5 * sum(s[0, k] + s[0, k + 1] for k in range(2, 7))
+    assert got[0] == want
+    assert trapezoid_window(s, [5], [5], 0.5)[0] == 0.0
