@@ -20,14 +20,18 @@
  *
  * Data layout of a propagation problem (one launch advances n_inst
  * independent instances over n_out * fine propagator steps each):
- *  - Superoperators are n x n real matrices (n <= 64), stored in a shared
- *    ELL pattern: column index ell_col[slot * 64 + row] (int32).  Slot 0
- *    of row r is always column r (the diagonal, value possibly 0) and
- *    ell_col[row] (slot 0) instead holds the number of used slots of that
- *    row; unused slots point at column r with value 0; values
- *    ell_val[((node * n_gen + g) * slots + slot) * 64 + row]; node ranges
+ *  - Superoperators are n x n real matrices (n <= 64; ssmx_propagate also
+ *    n <= 512), stored in a shared ELL pattern with row stride
+ *    NP = 64 * ceil(n / 64) (64 for n <= 64): column index
+ *    ell_col[slot * NP + row] (int32).  Slot 0 of row r is always column r
+ *    (the diagonal, value possibly 0) and ell_col[row] (slot 0) instead
+ *    holds the number of used slots of that row; unused slots point at
+ *    column r with value 0; values
+ *    ell_val[((node * n_gen + g) * slots + slot) * NP + row]; node ranges
  *    over n_node for the rotating frame (node-wise interaction-picture
- *    generators) and is 0 otherwise.
+ *    generators) and is 0 otherwise.  Readout rows (obs) and eigen-step
+ *    factors (eig) use the same stride NP.  For n > 64 the pattern may
+ *    use at most 32 slots and the stack at most 4 matrices.
  *  - One propagator step of size h applies exp(A_{J-1}) first and exp(A_0)
  *    last, with A_j = h sum_g (sum_s w[j,s] c[s,g]) G_g, or in the rotating
  *    frame A_j = sum_s sum_g (G~[s,g] c[s,g]) (h w[j,s]) followed by the
@@ -60,7 +64,9 @@ extern "C" {
 #define SSMX_READ_INTEGRAL 8 /* readout: trapezoid integral of obs rows */
 
 /* limits */
-#define SSMX_MAX_N 64
+#define SSMX_MAX_N 64       /* one wavefront per instance */
+#define SSMX_MAX_N_WIDE 512 /* ssmx_propagate only: one workgroup of
+                               ceil(n / 64) wavefronts per instance */
 #define SSMX_MAX_GEN 8
 #define SSMX_MAX_EXP 6
 #define SSMX_MAX_NODE 3
@@ -68,7 +74,8 @@ extern "C" {
 #define SSMX_MAX_TAYLOR 40
 
 typedef struct ssmx_propagate_desc {
-    int32_t n;        /* superoperator dimension, 1..64 */
+    int32_t n;        /* superoperator dimension, 1..64 (ssmx_propagate:
+                         1..512) */
     int32_t n_inst;   /* independent instances (one wavefront each) */
     int32_t n_gen;    /* K generators in the ELL pattern, 1..8 */
     int32_t slots;    /* ELL slots per row, 1..64 */
@@ -115,17 +122,17 @@ typedef struct ssmx_wave {
 /*
  * Propagate n_inst instances (one wavefront each).
  *
- * ell_col    int32 [slots * 64]
- * ell_val    double [(n_node or 1) * n_gen * slots * 64]
+ * ell_col    int32 [slots * NP]
+ * ell_val    double [(n_node or 1) * n_gen * slots * NP]
  * cf_weights double [n_exp * n_node]     (row j: weights of exponential j)
  * cf_nodes   double [n_node]             (Gauss-Legendre nodes on [0, 1])
  * coef       table mode: double [n_inst? * n_out * fine * n_node * n_gen]
  *            ordered (instance, i, r, s, g); wave mode: ssmx_wave [n_inst]
  * segments   wave mode: ssmx_segment [...]; table mode: unused (NULL)
  * rho0       double [n_inst * n]
- * eig        rotating: double [2 * 64] per-row (own, partner) factors of
+ * eig        rotating: double [2 * NP] per-row (own, partner) factors of
  *            exp(L0 h) - I; partner of row r is row r ^ 1
- * obs        double [n_obs * 64] readout rows (obs readouts, n_obs <= 2)
+ * obs        double [n_obs * NP] readout rows (obs readouts, n_obs <= 2)
  * obs_offset double [n_inst * n_obs] added to each obs readout (or NULL)
  * out        READ_FULL: double [n_inst * n_out * n]
  *            obs: double [n_inst * n_out * n_obs]

@@ -18,6 +18,7 @@ SSMX_READ_FULL = 4
 SSMX_READ_INTEGRAL = 8
 
 MAX_N = 64
+MAX_N_WIDE = 512
 MAX_GEN = 8
 MAX_EXP = 6
 MAX_NODE = 3

@@ -13,6 +13,15 @@
   CF4:2, readout of the fluorescence rho33 + rho44 + rho55.  Full
   (coherent) vectorisation, n = 49 (``use_incoherent=False``); n = 19 with
   the compressed one.
+* ``odmr21`` - the ``odmr7`` model with a 14N nucleus (I = 1) on every
+  orbital block, 21 levels: quadrupole -4.95 MHz, A_par -2.16 MHz,
+  A_perp -2.7 MHz (ground and excited triplets), nuclear gyromagnetic
+  ratio 3.077 MHz/T, zero static field (so the 2.86-2.88 GHz window holds
+  the three hyperfine lines of both ms = 0 <-> +-1 transitions), Rabi
+  0.5 MHz, 512 frequencies in 2.86-2.88 GHz, start in ground ms = 0 with
+  the nucleus maximally mixed, 5 us at 2 ns readout with 100 fine steps
+  (h = 20 ps, 250,000 steps), readout of the fluorescence (every excited
+  level).  n = 441 (coherent vectorisation) or 171 (compressed).
 
 * ``odmr7_gradient`` - the ``odmr7`` sweep (n = 49) with a piecewise-
   constant microwave amplitude (200 segments of 10 ns, Rabi frequencies
@@ -55,6 +64,26 @@ def drive_amplitude(rabi_hz):
 
 def _nv7_description(bz=5e-3):
     spins, spin_int, groups = models.nv_7(np.array([0.0, 0.0, bz]), params)
+    spins[0][0]["D"] = math.tau * 2.87e9
+    spins[1][0]["D"] = math.tau * 1.42e9
+    groups[((0, 0), (1, 0))] = {"rel": 65e6}
+    groups[((1, 0), (2, 0))] = {"s_gets_0": 8e6, "s_gets_1": 50e6}
+    groups[((2, 0), (0, 0))] = {"0_gets_s": 1e6, "1_gets_s": 0.7e6}
+    return spins, spin_int, groups
+
+
+#: 14N constants of ``odmr21`` (Hz, Hz/T)
+N14_QUADRUPOLE = -4.95e6
+N14_A_PAR = -2.16e6
+N14_A_PERP = -2.7e6
+N14_GAMMA = 3.077e6
+
+
+def _nv21_description(bz=0.0):
+    spins, spin_int, groups = models.nv_21(
+        np.array([0.0, 0.0, bz]), params, quadrupole=N14_QUADRUPOLE,
+        hyperfine_longitudinal=N14_A_PAR, hyperfine_transverse=N14_A_PERP,
+        nuclear_g=N14_GAMMA / params.fundamental.nuclear_gyro)
     spins[0][0]["D"] = math.tau * 2.87e9
     spins[1][0]["D"] = math.tau * 1.42e9
     groups[((0, 0), (1, 0))] = {"rel": 65e6}
@@ -150,6 +179,17 @@ def odmr7(count=4096, use_incoherent=False):
         _nv7_description(), np.linspace(2.77e9, 2.97e9, 4096)[:count],
         3e6, 5e6 / 65e6, 2e-6, 2e-9, 100, obs_levels=[3, 4, 5],
         rho0_diag={0: 1 / 3, 1: 1 / 3, 2: 1 / 3},
+        use_incoherent=use_incoherent)
+
+
+def odmr21(count=512, use_incoherent=False):
+    """Hyperfine-resolved sweep; levels: ground (ms, mI) 0-8 electron-major,
+    excited 9-17, singlet 18-20."""
+    return Workload(
+        "odmr21-nv14n-n441" if not use_incoherent else "odmr21-nv14n-n171",
+        _nv21_description(), np.linspace(2.86e9, 2.88e9, 512)[:count],
+        0.5e6, 5e6 / 65e6, 5e-6, 2e-9, 100, obs_levels=range(9, 18),
+        rho0_diag={3: 1 / 3, 4: 1 / 3, 5: 1 / 3},
         use_incoherent=use_incoherent)
 
 

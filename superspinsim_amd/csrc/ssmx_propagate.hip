@@ -45,7 +45,7 @@ using ssmx::wave_coef;
 using ssmx::wave_exchange;
 using ssmx::wave_sum;
 
-constexpr int kWavesPerBlock = 4;
+constexpr int kWavesPerBlock = 8;
 
 // 1/q for the Taylor recursion, q = 0..40 (entry 0 unused).
 __constant__ double kInv[SSMX_MAX_TAYLOR + 1] = {
@@ -370,8 +370,9 @@ int validate(const ssmx_propagate_desc *d)
 {
     if (!d)
         return set_error(SSMX_EINVAL, "null descriptor");
-    if (d->n < 1 || d->n > SSMX_MAX_N)
-        return set_error(SSMX_EINVAL, "n=%d outside 1..%d", d->n, SSMX_MAX_N);
+    if (d->n < 1 || d->n > SSMX_MAX_N_WIDE)
+        return set_error(SSMX_EINVAL, "n=%d outside 1..%d", d->n,
+                         SSMX_MAX_N_WIDE);
     if (d->n_inst < 1)
         return set_error(SSMX_EINVAL, "n_inst=%d < 1", d->n_inst);
     if (d->n_gen < 1 || d->n_gen > SSMX_MAX_GEN)
@@ -413,10 +414,34 @@ int validate(const ssmx_propagate_desc *d)
         return set_error(SSMX_EINVAL, "partner flag must be 0 or 1");
     if (d->ckpt_every < 0)
         return set_error(SSMX_EINVAL, "ckpt_every=%d < 0", d->ckpt_every);
+    if (d->n > SSMX_MAX_N) {
+        // one row per thread: at most 32 slots, and one instance's staged
+        // generator values plus its exchange buffers within 160 KiB of LDS
+        if (d->slots > 32)
+            return set_error(SSMX_EINVAL, "slots=%d > 32 for n > 64",
+                             d->slots);
+        const long np = (long)(d->n + kWave - 1) / kWave * kWave;
+        const long slotp = d->slots + (d->slots & 1);
+        const long lds = (M * slotp * np + 2 * np + np / kWave) * 8;
+        if (lds > 160 * 1024)
+            return set_error(SSMX_EINVAL,
+                             "n=%d with %d matrices x %d slots needs %ld "
+                             "bytes of LDS (> 160 KiB)",
+                             d->n, M, d->slots, lds);
+    }
     return SSMX_OK;
 }
 
 }  // namespace
+
+// n > 64: one workgroup per instance (ssmx_wide.hip).
+int ssmx_wide_propagate(const ssmx_propagate_desc *desc,
+                        const int32_t *ell_col, const double *ell_val,
+                        const double *cf_weights, const double *cf_nodes,
+                        const void *coef, const ssmx_segment *segments,
+                        const double *rho0, const double *eig,
+                        const double *obs, const double *obs_offset,
+                        double *out, double *state_out, hipStream_t stream);
 
 extern "C" {
 
@@ -474,6 +499,10 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
     p.out = out;
     p.state_out = state_out;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (desc->n > SSMX_MAX_N)
+        return ssmx_wide_propagate(desc, ell_col, ell_val, cf_weights,
+                                   cf_nodes, coef, segments, rho0, eig, obs,
+                                   obs_offset, out, state_out, s);
 
     switch (desc->slots) {
     case 1: case 2: return launch<2>(p, s);

@@ -42,6 +42,11 @@ def _ptr(t):
 # ELL packing
 # ---------------------------------------------------------------------------
 
+def padded_rows(n):
+    """Row stride of the device tables: 64 * ceil(n / 64)."""
+    return 64 * ((int(n) + 63) // 64)
+
+
 def pair_permutation(pattern):
     """Basis order that puts coupled index pairs at (2k, 2k + 1).
 
@@ -71,8 +76,8 @@ class EllStack:
     ``mats`` has shape (M, n, n).  Slot 0 of row r is always column r (the
     diagonal); with ``partner`` slot 1 is always column r ^ 1; the other
     non-zeros of the union fill slots 2.. in column order.  ``col`` is int32
-    (slots, 64), ``val`` float64 (M, slots, 64); unused slots point at
-    column r with value 0, rows >= n are all zero.
+    (slots, NP), ``val`` float64 (M, slots, NP) with NP = 64 ceil(n / 64);
+    unused slots point at column r with value 0, rows >= n are all zero.
     """
 
     def __init__(self, mats, partner=True):
@@ -81,8 +86,10 @@ class EllStack:
             raise ValueError(f"generator stack must be (M, n, n), got "
                              f"{mats.shape}")
         M, n, _ = mats.shape
-        if n > _lib.MAX_N:
-            raise ValueError(f"superoperator dimension {n} > {_lib.MAX_N}")
+        if n > _lib.MAX_N_WIDE:
+            raise ValueError(f"superoperator dimension {n} > "
+                             f"{_lib.MAX_N_WIDE}")
+        NP = padded_rows(n)
         union = np.any(mats != 0, axis=0)
         fixed = 2 if partner else 1
         rows = []
@@ -91,8 +98,8 @@ class EllStack:
             rows.append([c for c in np.nonzero(union[r])[0]
                          if c not in special])
         slots = fixed + max(len(c) for c in rows)
-        col = np.tile(np.arange(64, dtype=np.int32), (slots, 1))
-        val = np.zeros((M, slots, 64), dtype=_f64)
+        col = np.tile(np.arange(NP, dtype=np.int32), (slots, 1))
+        val = np.zeros((M, slots, NP), dtype=_f64)
         for r in range(n):
             val[:, 0, r] = mats[:, r, r]
             if partner:
@@ -108,6 +115,7 @@ class EllStack:
         for r in range(n):
             col[0, r] = fixed + len(rows[r])
         self.n = n
+        self.npad = NP
         self.slots = slots
         self.partner = bool(partner)
         self.col = col
@@ -290,8 +298,8 @@ def eig_lane_factors(d_step, s_step, n, residual=True):
     With ``residual`` the step factors are exp(L0 h) - I (expm1 forms);
     otherwise they are exp(L0 h) and 1 is subtracted here.
     """
-    own = np.zeros(64, dtype=_f64)
-    part = np.zeros(64, dtype=_f64)
+    own = np.zeros(padded_rows(n), dtype=_f64)
+    part = np.zeros(padded_rows(n), dtype=_f64)
     nd = d_step.shape[0]
     shift = 0.0 if residual else 1.0
     for y in range(nd):
@@ -458,7 +466,7 @@ class Propagator:
             t0=float(t0), dt=float(dt))
         obs_rows = None
         if obs is not None:
-            rows = np.zeros((n_obs, 64), dtype=_f64)
+            rows = np.zeros((n_obs, self.ell.npad), dtype=_f64)
             rows[:, :self.n] = np.asarray(obs)[:, self.perm]
             obs_rows = device_tensor(rows)
         rho0 = rho0[:, self.d_perm].contiguous()

@@ -90,7 +90,8 @@ def _desc(**kw):
 
 
 @pytest.mark.parametrize("bad", [
-    dict(n=0), dict(n=65), dict(n_inst=0), dict(n_gen=9), dict(slots=0),
+    dict(n=0), dict(n=513), dict(n=100, slots=33),
+    dict(n=441, n_gen=8, slots=14), dict(n_inst=0), dict(n_gen=9), dict(slots=0),
     dict(n_exp=7), dict(n_node=4), dict(taylor=0), dict(taylor=41),
     dict(substeps=3), dict(h=0.0), dict(first_out=-1), dict(partner=2),
     dict(ckpt_every=-1),

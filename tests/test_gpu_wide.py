@@ -1,0 +1,134 @@
+"""Superoperators wider than one wavefront (n > 64) on the GPU.
+
+The 21-level NV + 14N model: n = 441 (coherent vectorisation, 7 wavefronts
+per instance, 2 instances per workgroup) and n = 171 (compressed, 3
+wavefronts, 5 instances per workgroup).
+
+* Short runs against the reference-algorithm oracle (numpy backend of the
+  restated CPU path): every component of every output state, and two
+  readout rows at once; 1e-10 relative to the largest |component|.
+  Instance counts leave idle slots in the last workgroup.
+* Full-length sweep points (250,000 steps): the coherent and compressed
+  vectorisations of the same physics give the same fluorescence (the
+  compressed one drops coherences between incoherently coupled blocks,
+  which stay zero), and the state keeps unit trace.
+* The integral readout equals the trapezoid of the per-step trace.
+"""
+
+import numpy as np
+import pytest
+
+from oracle.reference_simulate import generate_simulator as oracle_gen
+from superspinsim_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _block_state(wl, seed):
+    """Random density matrix, block-diagonal over the three orbital blocks
+    (ground 0-8, excited 9-17, singlet 18-20)."""
+    rng = np.random.default_rng(seed)
+    rho = np.zeros((21, 21), dtype=complex)
+    for lo, hi in ((0, 9), (9, 18), (18, 21)):
+        k = hi - lo
+        a = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+        rho[lo:hi, lo:hi] = a @ a.conj().T
+    return rho / np.trace(rho).real
+
+
+def _short_workload(count, use_incoherent, rabi_hz=20e6):
+    wl = configs.odmr21(count, use_incoherent=use_incoherent)
+    wl.b1 = configs.drive_amplitude(rabi_hz)
+    wl.fine = 4
+    wl.time_step = 1e-10                 # h = 25 ps
+    wl.rho0_matrix = _block_state(wl, 5)
+    wl.rho0 = wl._flat(wl.rho0_matrix)
+    return wl
+
+
+def _oracle_states(wl, wave, n_out):
+    sim = oracle_gen(wave.sampler(), wl.generators, wl.vmap,
+                     backend="numpy",
+                     number_of_exponentials=wl.number_of_exponentials,
+                     number_of_fine_divisions=wl.fine)
+    _, dens = sim(wl.rho0_matrix, 0.0, n_out * wl.time_step * (1 + 1e-12),
+                  wl.time_step)
+    return np.stack([np.array([
+        (d[y, x].real if c == 0 else d[y, x].imag)
+        for (y, x, c) in wl.vmap]) for d in dens])
+
+
+@pytest.mark.parametrize("use_incoherent,count", [(True, 7), (False, 3)])
+def test_wide_states_vs_oracle(gpu, use_incoherent, count):
+    wl = _short_workload(count, use_incoherent)
+    n_out = 2
+    waves = wl.waveforms()
+    sim = wl.simulator()
+    print(f"{wl.name}: {count} instances x {n_out} outputs", flush=True)
+    t_end = n_out * wl.time_step * (1 + 1e-12)
+    full = sim.run(waves, wl.rho0, 0.0, t_end, wl.time_step,
+                   read="full").cpu().numpy()
+    obs = np.stack([wl.obs[0], np.zeros(wl.n)])
+    obs[1, [wl._index(k, k) for k in (3, 4, 5)]] = 1.0
+    two = sim.run(waves, wl.rho0, 0.0, t_end, wl.time_step, obs=obs,
+                  read="obs").cpu().numpy()
+    assert sim.last_plan["n"] == wl.n > 64
+    for k, w in enumerate(waves):
+        ref = _oracle_states(wl, w, n_out)
+        scale = np.max(np.abs(ref))
+        err = np.max(np.abs(full[k] - ref)) / scale
+        print(f"  f={wl.frequencies[k]:.6e}: state rel err {err:.3e}")
+        assert err <= TOL
+        err_obs = np.max(np.abs(two[k] - ref @ obs.T)) / scale
+        assert err_obs <= TOL
+
+
+def test_wide_vectorisations_agree_full_length(gpu):
+    """250,000 steps at four sweep points, n = 441 vs n = 171."""
+    freqs = np.array([2.8656e9, 2.8678e9, 2.8700e9, 2.8722e9])
+    traces = []
+    for inc in (False, True):
+        wl = configs.odmr21(4, use_incoherent=inc)
+        wl.frequencies = freqs
+        sim = wl.simulator()
+        waves = wl.waveforms()
+        print(f"{wl.name}: 4 x {wl.n_out} outputs x {wl.fine}", flush=True)
+        out = sim.run(waves, wl.rho0, 0.0, wl.time_end, wl.time_step,
+                      obs=wl.obs, read="obs")
+        traces.append(out[:, :, 0].cpu().numpy())
+        assert sim.last_plan["steps"] == 250000
+    a, b = traces
+    err = np.max(np.abs(a - b)) / np.max(np.abs(a))
+    print(f"  n441 vs n171 rel diff {err:.3e}")
+    assert err <= TOL
+    assert np.all(np.isfinite(a)) and np.max(a) > 1e-3
+
+
+def test_wide_trace_preserved(gpu):
+    wl = configs.odmr21(2, use_incoherent=True)
+    sim = wl.simulator()
+    waves = wl.waveforms()
+    n_out = 50
+    full = sim.run(waves, wl.rho0, 0.0, n_out * wl.time_step * (1 + 1e-12),
+                   wl.time_step, read="full").cpu().numpy()
+    diag = [wl._index(k, k) for k in range(21)]
+    tr = full[:, :, diag].sum(axis=2)
+    assert np.max(np.abs(tr - 1.0)) < 1e-12
+
+
+def test_wide_integral_is_trapezoid(gpu):
+    wl = configs.odmr21(3, use_incoherent=True)
+    sim = configs.BatchSimulator(wl.generators, 2, 1)
+    waves = wl.waveforms()
+    h = wl.time_step / wl.fine
+    n = 300
+    trace = sim.run(waves, wl.rho0, 0.0, n * h * (1 + 1e-12), h,
+                    obs=wl.obs, read="obs")[:, :, 0].cpu().numpy()
+    integ = sim.run(waves, wl.rho0, 0.0, n * h * (1 + 1e-12), h,
+                    obs=wl.obs, read="integral")[:, 0].cpu().numpy()
+    f0 = wl.obs[0] @ wl.rho0
+    full = np.concatenate([np.full((3, 1), f0), trace], axis=1)
+    ref = np.sum(0.5 * h * (full[:, 1:] + full[:, :-1]), axis=1)
+    assert np.max(np.abs(integ - ref) / np.abs(ref)) < 1e-13
