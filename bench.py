@@ -18,6 +18,11 @@ The protocol leg runs 16384 pulsed Ramsey / Hahn-echo protocols of 58k-742k
 steps in one persistent work-queue launch (autotuned shape, replayed from a
 HIP graph) and reports propagator steps/s over the whole batch.
 
+The hyperfine leg runs the 21-level NV + 14N sweep (512 points, 250,000
+steps each, fluorescence every 2 ns) at n = 441 and at n = 171 through the
+wide propagator (several wavefronts per instance), with a bounded CPU sample
+of the reference algorithm at n = 441.
+
 Prints one JSON line (rank 0, N = 1 GPU).
 """
 
@@ -72,6 +77,14 @@ def parse():
     p.add_argument("--protocol-count", type=int, default=16384)
     p.add_argument("--protocol-steps", type=int, default=1,
                    help="timed graph replays of the protocol batch")
+    p.add_argument("--no-hyperfine", action="store_true",
+                   help="skip the 21-level hyperfine (n = 441) leg")
+    p.add_argument("--hyperfine-only", action="store_true")
+    p.add_argument("--hyperfine-instances", type=int, default=512)
+    p.add_argument("--hyperfine-outputs", type=int, default=None,
+                   help="truncate the hyperfine sweep to this many output "
+                        "intervals (profiling runs)")
+    p.add_argument("--hyperfine-cpu-steps", type=int, default=16)
     return p.parse_args()
 
 
@@ -84,13 +97,9 @@ def workload(name, count):
     return configs.odmr3(count or 256)
 
 
-def cpu_baseline(wl, outputs):
-    """Reference algorithm (oracle/cpu_ref.c, OpenMP) on one sweep point.
-
-    Sample: the first ``outputs`` output intervals of the first sweep
-    point (``outputs * fine`` propagator steps), parallel over output
-    intervals like the reference's CPU path.
-    """
+def _cpu_sample(wl, outputs, threads):
+    """Seconds for ``outputs`` output intervals of the reference algorithm
+    (oracle/cpu_ref.c) on the first sweep point of ``wl``."""
     import numpy as np
     from oracle.reference_simulate import generate_simulator as oracle_gen
     w = wl.waveform(wl.frequencies[0])
@@ -102,8 +111,6 @@ def cpu_baseline(wl, outputs):
         out[:, 4] = w.base[4]
         return out
 
-    threads = min(len(os.sched_getaffinity(0)),
-                  int(os.environ.get("OMP_NUM_THREADS", "1024")))
     sim = oracle_gen(None, wl.generators, wl.vmap, backend="c",
                      number_of_exponentials=wl.number_of_exponentials,
                      number_of_fine_divisions=wl.fine,
@@ -112,7 +119,23 @@ def cpu_baseline(wl, outputs):
     sim(wl.rho0_matrix, 0.0, 2 * wl.time_step, wl.time_step)  # warm
     t0 = time.perf_counter()
     sim(wl.rho0_matrix, 0.0, t_end * (1 + 1e-12), wl.time_step)
-    elapsed = time.perf_counter() - t0
+    return time.perf_counter() - t0
+
+
+def _threads():
+    return min(len(os.sched_getaffinity(0)),
+               int(os.environ.get("OMP_NUM_THREADS", "1024")))
+
+
+def cpu_baseline(wl, outputs):
+    """Reference algorithm (oracle/cpu_ref.c, OpenMP) on one sweep point.
+
+    Sample: the first ``outputs`` output intervals of the first sweep
+    point (``outputs * fine`` propagator steps), parallel over output
+    intervals like the reference's CPU path.
+    """
+    threads = _threads()
+    elapsed = _cpu_sample(wl, outputs, threads)
     steps = outputs * wl.fine
     return {"value": steps / elapsed, "unit": "propagator steps/s",
             "cores": threads, "kind": "port",
@@ -248,6 +271,94 @@ def protocol_leg(args):
     }
 
 
+def hyperfine_leg(args):
+    """21-level NV + 14N sweep (n = 441 and 171) through the wide kernel."""
+    import numpy as np
+    import torch
+    from superspinsim_amd import configs
+    from superspinsim_amd.propagator import device_tensor
+    from superspinsim_amd.sweep import pack_waveforms
+    legs = {}
+    for inc in (False, True):
+        wl = configs.odmr21(args.hyperfine_instances, use_incoherent=inc)
+        if args.hyperfine_outputs:
+            wl.n_out = args.hyperfine_outputs
+            wl.time_end = wl.n_out * wl.time_step * (1 + 1e-12)
+        sim = wl.simulator()
+        waves = wl.waveforms()
+        n_inst = len(waves)
+        # plan and device tables from a one-interval run (untimed)
+        sim.run(waves, wl.rho0, 0.0, wl.time_step * (1 + 1e-12),
+                wl.time_step, obs=wl.obs, read="obs")
+        plan = dict(sim.last_plan)
+        prop, program, m, sub = sim.plan(waves, wl.time_step)
+        d_waves, d_segs = pack_waveforms(sim.kernel_waves(waves, program),
+                                         len(program))
+        d_rho = device_tensor(np.repeat(wl.rho0[None], n_inst, axis=0))
+        out = torch.empty((n_inst, wl.n_out, 1), dtype=torch.float64,
+                          device="cuda")
+
+        def launch():
+            prop.run(n_inst=n_inst, fine=wl.fine, n_out=wl.n_out,
+                     h=wl.time_step / wl.fine, t0=0.0, dt=wl.time_step,
+                     rho0=d_rho, waves=d_waves, segments=d_segs, taylor=m,
+                     substeps=sub, read="obs", obs=wl.obs, out=out)
+
+        steps = wl.n_out * wl.fine
+        print(f"hyperfine workload {wl.name}: {n_inst} x {steps} steps, "
+              f"n={wl.n}, m={m}, s={sub}, slots={plan['slots']}",
+              file=sys.stderr, flush=True)
+        launch()                                    # warm
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        t0 = time.perf_counter()
+        ev[0].record(stream)
+        launch()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_s = ev[0].elapsed_time(ev[1]) / 1e3
+        plan["steps"] = steps
+        flops = sim.step_flops(plan) * n_inst * steps
+        fl = out[:, :, 0].cpu().numpy()
+        print(f"  {wall:.3f} s, {n_inst * steps / wall:.3e} steps/s",
+              file=sys.stderr, flush=True)
+        legs[wl.name] = {
+            "instances": n_inst,
+            "superoperator_n": wl.n,
+            "steps_per_instance": steps,
+            "taylor_degree": m,
+            "substeps": sub,
+            "ell_slots": plan["slots"],
+            "nnz": plan["nnz"],
+            "s_per_sweep": wall,
+            "kernel_s": kernel_s,
+            "steps_per_s": n_inst * steps / wall,
+            "flops_per_step": sim.step_flops(plan),
+            "achieved_tflops": flops / kernel_s / 1e12,
+            "frac_fp64_peak": flops / kernel_s / 1e12 / PEAK_FP64_TFLOPS,
+            "finite": bool(np.all(np.isfinite(fl))),
+            "fluorescence_max": float(fl.max()),
+        }
+        if not inc and not args.no_cpu and args.hyperfine_cpu_steps:
+            # reference algorithm at n = 441: one propagator step per
+            # output interval (its cost per step does not depend on h)
+            cw = configs.odmr21(1)
+            cw.fine = 1
+            cw.time_step = wl.time_step / wl.fine
+            threads = _threads()
+            k = args.hyperfine_cpu_steps
+            el = _cpu_sample(cw, k, threads)
+            legs[wl.name]["cpu_baseline"] = {
+                "value": k / el, "unit": "propagator steps/s",
+                "cores": threads, "kind": "port",
+                "sample": f"{k} propagator steps of one {wl.name} sweep "
+                          f"point ({el:.1f} s), reference algorithm "
+                          "(oracle/cpu_ref.c)"}
+    return legs
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -273,6 +384,9 @@ def main():
         return
     if args.protocols_only:
         print(json.dumps({"protocols": protocol_leg(args)}))
+        return
+    if args.hyperfine_only:
+        print(json.dumps({"hyperfine": hyperfine_leg(args)}))
         return
 
     wl = workload(args.config, args.instances)
@@ -387,6 +501,8 @@ def main():
         result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]
     if not args.no_protocols and args.config == "odmr7":
         result["protocols"] = protocol_leg(args)
+    if not args.no_hyperfine and args.config == "odmr7":
+        result["hyperfine"] = hyperfine_leg(args)
     if not args.no_cpu:
         outputs = args.cpu_outputs or (128 if wl.n > 20 else 999)
         result["cpu_baseline"] = cpu_baseline(wl, outputs)

@@ -30,8 +30,10 @@
  *    ell_val[((node * n_gen + g) * slots + slot) * NP + row]; node ranges
  *    over n_node for the rotating frame (node-wise interaction-picture
  *    generators) and is 0 otherwise.  Readout rows (obs) and eigen-step
- *    factors (eig) use the same stride NP.  For n > 64 the pattern may
- *    use at most 32 slots and the stack at most 4 matrices.
+ *    factors (eig) use the same stride NP.  For n > 64 the pattern may use
+ *    at most 32 slots, and the staged values (matrices x slots x NP
+ *    doubles) plus 8 KiB of exchange buffers must fit 160 KiB of LDS; the
+ *    used-slot counts let each wavefront skip slots none of its rows use.
  *  - One propagator step of size h applies exp(A_{J-1}) first and exp(A_0)
  *    last, with A_j = h sum_g (sum_s w[j,s] c[s,g]) G_g, or in the rotating
  *    frame A_j = sum_s sum_g (G~[s,g] c[s,g]) (h w[j,s]) followed by the

@@ -56,6 +56,17 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Largest v over the wavefront, returned wave-uniform (scalar register).
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, kWave);
+        v = o > v ? o : v;
+    }
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Exchange barrier of one wavefront: LDS writes before it are visible to
 // LDS reads after it (LDS instructions of a wavefront execute in order).
 __device__ __forceinline__ void wave_exchange()

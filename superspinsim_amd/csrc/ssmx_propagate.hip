@@ -415,14 +415,16 @@ int validate(const ssmx_propagate_desc *d)
     if (d->ckpt_every < 0)
         return set_error(SSMX_EINVAL, "ckpt_every=%d < 0", d->ckpt_every);
     if (d->n > SSMX_MAX_N) {
-        // one row per thread: at most 32 slots, and one instance's staged
-        // generator values plus its exchange buffers within 160 KiB of LDS
+        // one row per thread: at most 32 slots, and the staged generator
+        // values plus one instance's two 512-row exchange buffers within
+        // 160 KiB of LDS
         if (d->slots > 32)
             return set_error(SSMX_EINVAL, "slots=%d > 32 for n > 64",
                              d->slots);
         const long np = (long)(d->n + kWave - 1) / kWave * kWave;
         const long slotp = d->slots + (d->slots & 1);
-        const long lds = (M * slotp * np + 2 * np + np / kWave) * 8;
+        const long lds =
+            (M * slotp * np + 2 * SSMX_MAX_N_WIDE + np / kWave) * 8;
         if (lds > 160 * 1024)
             return set_error(SSMX_EINVAL,
                              "n=%d with %d matrices x %d slots needs %ld "

@@ -13,10 +13,18 @@
 //    wavefront) is a DPP quad permutation;
 //  - other slots are read from an LDS buffer of the term; two buffers
 //    alternate, so one workgroup barrier per term orders every read of a
-//    term before the next write of the same buffer.
+//    term before the next write of the same buffer.  The buffers are a
+//    fixed 4 KiB apart, so the gather addresses are computed once and the
+//    buffer is selected by an immediate offset.
 // A_j's row is assembled once per exponential from the staged values with
 // the 64-row kernel's roundings, so both kernels give the same states for
 // the same problem.
+// Each wavefront runs an exponential specialised to the slots its widest row
+// uses (the host orders row pairs by width, so most wavefronts need far
+// fewer than the widest row of the matrix).  Every wavefront of a workgroup
+// meets every other at each product's barrier, so the product's instruction
+// count, times the wavefronts per SIMD, sets the pace: no address
+// arithmetic and no work on unused slots in the product.
 //
 // Dense FP64 MFMA is not used: at n = 441 the union of the generators has
 // 3,677 non-zeros (at most 14 per row), so a dense product would do about
@@ -33,10 +41,14 @@ using ssmx::read_lane;
 using ssmx::set_error;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
+using ssmx::wave_max;
 using ssmx::wave_sum;
 
 // largest workgroup: 16 wavefronts (1024 threads)
 constexpr int kMaxBlockWaves = 16;
+// stride between an instance's two exchange buffers (doubles): one buffer
+// holds up to 512 rows; 4 KiB fits a ds_read immediate offset
+constexpr int kBuf = SSMX_MAX_N_WIDE;
 
 __constant__ double kInvW[SSMX_MAX_TAYLOR + 1] = {
     0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
@@ -99,6 +111,137 @@ __device__ double wide_factor(const WParams &p, const double *coef_i,
     return d.h * cw;
 }
 
+// Wave-uniform inputs of one exponential.
+struct ExpArgs {
+    const double *lv;       // staged values [m][slot / 2][row][slot % 2]
+    int NP, M, base;        // rows, matrices, first factor lane
+    int substeps, taylor;
+    double inv_sub;
+    bool partner;
+};
+
+// y = A v for this thread's row, reading buffer P (0 or 1): v is written to
+// the row's own entry, then WS slots are combined (slot 0 diagonal, slot 1
+// partner, others gathered).  Two accumulation chains (even and odd
+// slots), as in ssmx_propagate.hip.
+template <int SLOTS, int WS, int P>
+__device__ __forceinline__ double wide_product(
+    const double (&a)[WS], double v, double *own,
+    const double *const (&gp)[SLOTS], bool partner)
+{
+    own[P * kBuf] = v;
+    __syncthreads();
+    double g[WS];
+    g[0] = v;
+    g[1] = partner ? swap_pair(v) : gp[1][P * kBuf];
+#pragma unroll
+    for (int sl = 2; sl < WS; ++sl)
+        g[sl] = gp[sl][P * kBuf];
+    double y0 = a[0] * g[0];
+    double y1 = a[1] * g[1];
+#pragma unroll
+    for (int sl = 2; sl < WS; ++sl) {
+        if (sl & 1)
+            y1 = fma(a[sl], g[sl], y1);
+        else
+            y0 = fma(a[sl], g[sl], y0);
+    }
+    return y0 + y1;
+}
+
+// x <- exp(A_j) x for this thread's row: assemble the row of A_j from the
+// staged values (slots < WS), then s substeps of the degree-m Taylor
+// action.  `parity` is the buffer the next product writes.
+template <int SLOTS, int WS>
+__device__ __forceinline__ double wide_exp(double x, const ExpArgs &e,
+                                           int row, double factor,
+                                           double inv_lane, double *own,
+                                           const double *const (&gp)[SLOTS],
+                                           int &parity)
+{
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    double a[WS];
+#pragma unroll
+    for (int sl = 0; sl < WS; ++sl)
+        a[sl] = 0.0;
+    for (int m = 0; m < e.M; ++m) {
+        const double f = read_lane(factor, e.base + m);
+        const double2 *v =
+            reinterpret_cast<const double2 *>(e.lv +
+                                              (size_t)m * SLOTP * e.NP) +
+            row;
+        double2 vv[WS / 2];
+#pragma unroll
+        for (int pr = 0; pr < WS / 2; ++pr)
+            vv[pr] = v[pr * e.NP];
+#pragma unroll
+        for (int pr = 0; pr < WS / 2; ++pr) {
+            a[2 * pr] = fma(f, vv[pr].x, a[2 * pr]);
+            a[2 * pr + 1] = fma(f, vv[pr].y, a[2 * pr + 1]);
+        }
+    }
+    if (e.substeps > 1) {
+#pragma unroll
+        for (int sl = 0; sl < WS; ++sl)
+            a[sl] *= e.inv_sub;
+    }
+    // term_t = (A term_{t-1}) / t; acc += term_t
+    auto taylor_term = [&](double y, int t, double &term, double &acc) {
+#pragma clang fp contract(off)
+        term = y * read_lane(inv_lane, t);
+        acc += term;
+    };
+    for (int sub = 0; sub < e.substeps; ++sub) {
+        double term = x;
+        double acc = x;
+        // products alternate buffers; the loop body is one pair (buffer 0,
+        // then 1), so both buffer offsets are immediates
+        int t = 1;
+        if (parity && t <= e.taylor) {
+            taylor_term(wide_product<SLOTS, WS, 1>(a, term, own, gp,
+                                                   e.partner),
+                        t, term, acc);
+            ++t;
+            parity = 0;
+        }
+        for (; t + 1 <= e.taylor; t += 2) {
+            taylor_term(wide_product<SLOTS, WS, 0>(a, term, own, gp,
+                                                   e.partner),
+                        t, term, acc);
+            taylor_term(wide_product<SLOTS, WS, 1>(a, term, own, gp,
+                                                   e.partner),
+                        t + 1, term, acc);
+        }
+        if (t <= e.taylor) {
+            taylor_term(wide_product<SLOTS, WS, 0>(a, term, own, gp,
+                                                   e.partner),
+                        t, term, acc);
+            parity = 1;
+        }
+        x = acc;
+    }
+    return x;
+}
+
+// wide_exp for the smallest even slot class WS >= ws (wave-uniform).
+template <int SLOTS, int WS = 2>
+__device__ __forceinline__ double wide_exp_dispatch(
+    int ws, double x, const ExpArgs &e, int row, double factor,
+    double inv_lane, double *own, const double *const (&gp)[SLOTS],
+    int &parity)
+{
+    if constexpr (WS >= SLOTS) {
+        return wide_exp<SLOTS, SLOTS>(x, e, row, factor, inv_lane, own, gp,
+                                      parity);
+    } else {
+        if (ws <= WS)
+            return wide_exp<SLOTS, WS>(x, e, row, factor, inv_lane, own, gp,
+                                       parity);
+        return wide_exp_dispatch<SLOTS, WS + 2>(ws, x, e, row, factor,
+                                                inv_lane, own, gp, parity);
+    }
+}
+
 template <int SLOTS>
 __global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
     WParams p, int W, int IPB)
@@ -125,12 +268,11 @@ __global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
     const int J = d.n_exp;
 
     // LDS: staged values [m][slot / 2][row][slot % 2], then per instance
-    // slot two exchange buffers of NP doubles, then one partial sum per
-    // wavefront
+    // slot two exchange buffers kBuf doubles apart, then one partial sum
+    // per wavefront
     double *lv = lds;
-    double *xb0 = lv + (size_t)M * SLOTP * NP + (size_t)ii * 2 * NP;
-    double *xb1 = xb0 + NP;
-    double *red = lv + (size_t)M * SLOTP * NP + (size_t)IPB * 2 * NP;
+    double *xb0 = lv + (size_t)M * SLOTP * NP + (size_t)ii * 2 * kBuf;
+    double *red = lv + (size_t)M * SLOTP * NP + (size_t)IPB * 2 * kBuf;
     {
         const int nval = M * SLOTP * NP;
         for (int e = tid; e < nval; e += blockDim.x) {
@@ -146,11 +288,18 @@ __global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
         __syncthreads();
     }
 
-    int col[SLOTS];
-    col[0] = row;
+    // slots this wavefront reads: its widest row (slot 0's column entry
+    // holds the row's used-slot count)
+    const int ws = wave_max(p.ell_col[row]);
+    // gather addresses (buffer 0); column indices are taken modulo the
+    // buffer size, so no table can address outside the instance's buffers
+    const double *gp[SLOTS];
+    gp[0] = xb0 + row;
 #pragma unroll
     for (int sl = 1; sl < SLOTS; ++sl)
-        col[sl] = sl < d.slots ? p.ell_col[sl * NP + row] : row;
+        gp[sl] = xb0 + ((sl < d.slots ? p.ell_col[sl * NP + row] : row) &
+                        (kBuf - 1));
+    double *own = xb0 + row;
     const double inv_lane = kInvW[lane <= SSMX_MAX_TAYLOR ? lane : 0];
 
     const int inst_c = active ? inst : 0;     // idle slots read instance 0
@@ -188,6 +337,15 @@ __global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
     const double half_h = 0.5 * d.h;
     double xint = 0.0;
     int parity = 0;
+    ExpArgs e;
+    e.lv = lv;
+    e.NP = NP;
+    e.M = M;
+    e.base = 0;
+    e.substeps = d.substeps;
+    e.taylor = d.taylor;
+    e.inv_sub = inv_sub;
+    e.partner = use_partner;
 
     // Sum of v over the instance's W wavefronts (every thread gets it).
     auto inst_sum = [&](double v) -> double {
@@ -200,31 +358,6 @@ __global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
             s += red[ii * W + k];
         __syncthreads();
         return s;
-    };
-
-    // y = A v (one Taylor product) with the exchange across the instance
-    auto product = [&](const double (&a)[SLOTS], double v) -> double {
-        double *xb = parity ? xb1 : xb0;
-        parity ^= 1;
-        xb[row] = v;
-        __syncthreads();
-        double g[SLOTS];
-        g[0] = v;
-        if (SLOTS > 1)
-            g[1] = use_partner ? swap_pair(v) : xb[col[1]];
-#pragma unroll
-        for (int sl = 2; sl < SLOTS; ++sl)
-            g[sl] = xb[col[sl]];
-        double y0 = a[0] * g[0];
-        double y1 = 0.0;
-#pragma unroll
-        for (int sl = 1; sl < SLOTS; ++sl) {
-            if (sl & 1)
-                y1 = fma(a[sl], g[sl], y1);
-            else
-                y0 = fma(a[sl], g[sl], y0);
-        }
-        return y0 + y1;
     };
 
     const int ckpt_every = p.state_out ? d.ckpt_every : 0;
@@ -258,41 +391,9 @@ __global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
             }
             const double x_prev = x;
             for (int j = J - 1; j >= 0; --j) {
-                double a[SLOTS];
-#pragma unroll
-                for (int sl = 0; sl < SLOTS; ++sl)
-                    a[sl] = 0.0;
-                const int base = (b * J + j) * M;
-                for (int m = 0; m < M; ++m) {
-                    const double f = read_lane(factor, base + m);
-                    const double2 *v =
-                        reinterpret_cast<const double2 *>(lv +
-                                                          (size_t)m * SLOTP *
-                                                              NP) +
-                        row;
-#pragma unroll
-                    for (int pr = 0; pr < SLOTP / 2; ++pr) {
-                        const double2 vv = v[pr * NP];
-                        a[2 * pr] = fma(f, vv.x, a[2 * pr]);
-                        if (2 * pr + 1 < SLOTS)
-                            a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
-                    }
-                }
-                if (d.substeps > 1) {
-#pragma unroll
-                    for (int sl = 0; sl < SLOTS; ++sl)
-                        a[sl] *= inv_sub;
-                }
-                for (int sub = 0; sub < d.substeps; ++sub) {
-                    double term = x;
-                    double acc = x;
-                    for (int t = 1; t <= d.taylor; ++t) {
-#pragma clang fp contract(off)
-                        term = product(a, term) * read_lane(inv_lane, t);
-                        acc += term;
-                    }
-                    x = acc;
-                }
+                e.base = (b * J + j) * M;
+                x = wide_exp_dispatch<SLOTS>(ws, x, e, row, factor, inv_lane,
+                                             own, gp, parity);
             }
             if (rotating) {
                 const double xp = swap_pair(x);
@@ -343,11 +444,11 @@ int launch(const WParams &p, hipStream_t stream)
     int ipb = kMaxBlockWaves / W;
     if (ipb > p.d.n_inst)
         ipb = p.d.n_inst;
-    while (ipb > 1 && staged + (size_t)ipb * (2 * NP + W) * sizeof(double) >
-                          160 * 1024)
+    while (ipb > 1 && staged + (size_t)ipb * (2 * kBuf + W) *
+                                   sizeof(double) > 160 * 1024)
         --ipb;
     const size_t lds =
-        staged + (size_t)ipb * (2 * NP + W) * sizeof(double);
+        staged + (size_t)ipb * (2 * kBuf + W) * sizeof(double);
     if (lds > 160 * 1024)
         return set_error(SSMX_EINVAL,
                          "n=%d: generator values need %zu bytes of LDS "

@@ -70,6 +70,32 @@ def pair_permutation(pattern):
     return np.array(order, dtype=np.int64)
 
 
+def width_order(pattern, partner=True):
+    """Order of aligned row pairs (2k, 2k + 1), widest first.
+
+    ``pattern`` is the (n, n) union pattern in the paired basis.  The width
+    of a row is its used ELL slots (diagonal, partner, other non-zeros);
+    pairs move as units, so a pair stays at (2k, 2k + 1), and an odd last
+    row stays last.  Returns ``order`` with ``new[k] = old[order[k]]``.
+    """
+    n = pattern.shape[0]
+    fixed = 2 if partner else 1
+    width = np.empty(n, dtype=np.int64)
+    for r in range(n):
+        others = np.count_nonzero(pattern[r])
+        others -= int(pattern[r, r])
+        if partner and (r ^ 1) < n:
+            others -= int(pattern[r, r ^ 1])
+        width[r] = fixed + others
+    units = n // 2
+    unit_width = np.maximum(width[0:2 * units:2], width[1:2 * units:2])
+    rank = np.argsort(-unit_width, kind="stable")
+    order = np.stack([2 * rank, 2 * rank + 1], axis=1).ravel()
+    if n % 2:
+        order = np.append(order, n - 1)
+    return order.astype(np.int64)
+
+
 class EllStack:
     """Generators packed in one ELL pattern (the union of their non-zeros).
 
@@ -363,6 +389,17 @@ class Propagator:
             perm = pair_permutation(np.any(flat != 0, axis=0))
         else:
             perm = np.arange(n0)
+        if n0 > _lib.MAX_N:
+            # several wavefronts per instance: widest pairs first, so that
+            # each wavefront reads only the slots its own rows use
+            pat = np.any(flat != 0, axis=0)[perm][:, perm]
+            perm = perm[width_order(pat)]
+            if eig is not None:
+                eig = np.asarray(eig, dtype=_f64)
+                half = eig.size // 2
+                own, part = eig[:half].copy(), eig[half:].copy()
+                own[:n0], part[:n0] = own[perm], part[perm]
+                eig = np.concatenate([own, part])
         self.perm = perm
         self.inv_perm = np.argsort(perm)
         flat = flat[:, perm][:, :, perm]

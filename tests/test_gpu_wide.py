@@ -13,13 +13,20 @@ wavefronts, 5 instances per workgroup).
   compressed one drops coherences between incoherently coupled blocks,
   which stay zero), and the state keeps unit trace.
 * The integral readout equals the trapezoid of the per-step trace.
+* The drop-in ``generate_simulator`` on the 21-level model (n = 171; the
+  rotating frame is requested and, as in the reference, refused because
+  L0 is singular) against the oracle.
 """
+
+import math
 
 import numpy as np
 import pytest
 
 from oracle.reference_simulate import generate_simulator as oracle_gen
 from superspinsim_amd import configs
+from superspinsim_amd.generate_simulator import generate_simulator
+from superspinsim_amd.sugar import prepare_simspins
 
 pytestmark = pytest.mark.gpu
 
@@ -132,3 +139,27 @@ def test_wide_integral_is_trapezoid(gpu):
     full = np.concatenate([np.full((3, 1), f0), trace], axis=1)
     ref = np.sum(0.5 * h * (full[:, 1:] + full[:, :-1]), axis=1)
     assert np.max(np.abs(integ - ref) / np.abs(ref)) < 1e-13
+
+
+def test_wide_dropin_generate_simulator_vs_oracle(gpu):
+    b1 = configs.drive_amplitude(20e6)
+    coeffs = [lambda t: b1 * math.cos(math.tau * 2.8678e9 * t),
+              lambda t: 0.0, lambda t: 0.0, lambda t: 5 / 65]
+    spins, si, gi = configs._nv21_description()
+    sampler, gens, vmap, kw = prepare_simspins(
+        coeffs, spins, si, gi, number_of_exponentials=2,
+        number_of_fine_divisions=4, use_rotating=True, use_incoherent=True)
+    assert gens[0].shape == (171, 171) and not kw["use_rotating"]
+    rho = np.zeros((21, 21))
+    for k in (3, 4, 5):
+        rho[k, k] = 1 / 3
+    rho[3, 4] = rho[4, 3] = 0.1
+    print("nv21 drop-in n=171, 3 outputs", flush=True)
+    t_dev, d_dev = generate_simulator(sampler, gens, vmap, **kw)(
+        rho, 0, 3e-10, 1e-10)
+    t_ref, d_ref = oracle_gen(sampler, gens, vmap, backend="numpy", **kw)(
+        rho, 0, 3e-10, 1e-10)
+    assert np.array_equal(t_dev, t_ref)
+    err = np.max(np.abs(d_dev - d_ref)) / np.max(np.abs(d_ref))
+    print(f"  rel err {err:.3e}")
+    assert err <= TOL

@@ -18,7 +18,8 @@ from superspinsim_amd.generate_generators import make_rotating
 from superspinsim_amd.generate_simulator import _flatten, _unflatten, tabulate
 from superspinsim_amd.propagator import (
     EllStack, coefficient_samples, eig_lane_factors, pair_permutation,
-    rotating_tables, taylor_plan, taylor_plan_measured, taylor_remainder)
+    rotating_tables, taylor_plan, taylor_plan_measured, taylor_remainder,
+    width_order)
 from superspinsim_amd.sweep import WAVE_DTYPE, Waveform, waveform_records
 
 
@@ -56,6 +57,45 @@ def test_pair_permutation_pairs_coherences():
                  if tuple(vm[k][:2]) == tuple(vm[k + 1][:2])
                  and vm[k][2] != vm[k + 1][2])
     assert paired == 21  # all off-diagonal (y, x) of the 7-level model
+
+
+@pytest.mark.parametrize("n", [9, 10, 171])
+def test_width_order_keeps_pairs_and_sorts_units(n):
+    rng = np.random.default_rng(n)
+    pat = rng.random((n, n)) < 0.08
+    pat |= np.eye(n, dtype=bool)
+    order = width_order(pat)
+    assert sorted(order) == list(range(n))
+    # aligned pairs move as units; an odd last row stays last
+    assert np.all(order[1:2 * (n // 2):2] == order[0:2 * (n // 2):2] + 1)
+    assert np.all(order[0:2 * (n // 2):2] % 2 == 0)
+    if n % 2:
+        assert order[-1] == n - 1
+    # widths (used ELL slots) of the reordered matrix: nonincreasing units
+    pm = pat[order][:, order]
+    ell = EllStack(pm[None].astype(float) + 0.0)
+    w = ell.col[0, :n]
+    units = np.maximum(w[0:2 * (n // 2):2], w[1:2 * (n // 2):2])
+    assert np.all(np.diff(units) <= 0)
+    # the reordering is a similarity: same matrix up to relabelling
+    assert np.array_equal(pm, pat[np.ix_(order, order)])
+
+
+def test_wide_propagator_orders_pairs_by_width():
+    """The 21-level model's slot counts per 64-row wavefront after the
+    width ordering (what the wide kernel reads)."""
+    wl = configs.odmr21(1)
+    mats = np.stack([wl.generators[0] + wl.excitation * wl.generators[4],
+                     wl.generators[1]])
+    pat = np.any(mats != 0, axis=0)
+    perm = pair_permutation(pat)
+    perm = perm[width_order(pat[perm][:, perm])]
+    ell = EllStack(mats[:, perm][:, :, perm])
+    per_wave = [int(ell.col[0, w * 64:(w + 1) * 64].max())
+                for w in range(ell.npad // 64)]
+    assert per_wave[0] == ell.slots == 14
+    assert per_wave == sorted(per_wave, reverse=True)
+    assert sum(per_wave) <= 64
 
 
 @pytest.mark.parametrize("norm", [0.01, 0.19, 0.8, 3.0, 40.0])
