@@ -215,6 +215,20 @@ int ssmx_propagate_queue(const ssmx_propagate_desc *desc,
  * wavefront recomputes segment p - 1 while a second one runs the adjoint of
  * segment p, exchanging states through `workspace`.
  *
+ * n > 64 (up to 512; one workgroup of ceil(n / 64) wavefronts per
+ * instance, recomputation then reverse sweep per segment): tables use the
+ * row stride NP = 64 ceil(n / 64) throughout, and
+ *  - ell_col/ell_val: the matrices' pattern extended by the derivative
+ *    tables' non-zeros (`slots` in all; the matrices' own columns first,
+ *    values 0 in the derivative-only slots); ell_col[row] holds the used
+ *    slots in bits 0-7 and the matrices' used slots in bits 8-15;
+ *    `reserved` = the matrices' slot count (<= 16; 0 = slots);
+ *  - ell_col_t: the transposed pattern of the matrices (<= 16 slots);
+ *    ell_val_t is not read (the values of A^T are taken from ell_val);
+ *  - dval_t: D_t[r][c] at the slots of ell_col, [n_theta][slots][NP];
+ *  - obs rows [n_obs][NP]; grad_gen must be -1, partner 1;
+ *  - workspace: n_inst * ckpt_every * n_exp * NP doubles.
+ *
  * No counterpart in the reference (it has no gradients); it differentiates
  * the path ssmx_propagate replaces (generate_simulator.py:1945-2116).
  */
@@ -244,7 +258,7 @@ typedef struct ssmx_adjoint_desc {
     int32_t theta_gen;  /* matrix the derivative tables belong to (or -1) */
     int32_t grad_gen;   /* matrix whose waveform segments get amplitude
                            gradients (or -1; needs SSMX_COEF_WAVE) */
-    int32_t reserved;   /* 0 */
+    int32_t reserved;   /* 0; n > 64: the matrices' slot count */
     int64_t coef_inst_stride;
     double h;
     double t0;
@@ -254,7 +268,7 @@ typedef struct ssmx_adjoint_desc {
 #define SSMX_ADJ_MAX_THETA 16
 
 /* Bytes of device workspace ssmx_adjoint needs: n_inst * 2 * ckpt_every *
- * n_exp * 64 doubles. */
+ * n_exp * 64 doubles (n > 64: n_inst * ckpt_every * n_exp * NP). */
 int ssmx_adjoint_workspace_bytes(const ssmx_adjoint_desc *desc,
                                  uint64_t *bytes);
 

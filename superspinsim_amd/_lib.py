@@ -27,6 +27,8 @@ MAX_TAYLOR = 40
 ADJ_MAX_TAYLOR = 10
 ADJ_MAX_THETA = 16
 ADJ_MAX_SLOTS = 32
+#: n > 64 adjoint: matrix and transposed slots staged per row
+ADJ_WIDE_MAX_SLOTS = 16
 
 
 class PropagateDesc(ctypes.Structure):

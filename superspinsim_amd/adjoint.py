@@ -87,6 +87,65 @@ def shared_adjoint_ell(fwd_col, mats, dmats):
     return col, val_f, val_t, dval_t
 
 
+def wide_adjoint_ell(ell, mats, dmats):
+    """Adjoint tables for n > 64 (several wavefronts per instance).
+
+    ``ell`` is the propagator's ``EllStack`` (the matrices' own pattern,
+    row stride NP); ``mats`` (M, n, n) and ``dmats`` (n_theta, n, n) are in
+    its basis.  Returns (col_e, val_e, col_t, dval_e, slots_f):
+
+    * ``col_e`` (slots_e, NP): row r lists the matrices' columns in the
+      propagator's slot order, then the other columns with a non-zero
+      derivative entry D_t[r][c]; slot 0 holds the used count | the
+      matrices' count << 8;
+    * ``val_e`` (M, slots_e, NP): G_m[r][c] (zero in derivative-only slots);
+    * ``col_t`` (slots_t, NP): the transposed pattern of the matrices
+      (slot 0 diagonal with the used count, slot 1 the partner r ^ 1);
+    * ``dval_e`` (n_theta, slots_e, NP): D_t[r][c] at the slots of col_e;
+    * ``slots_f``: slots of the matrices' own pattern (``ell.slots``).
+    """
+    M, n, _ = mats.shape
+    NP = ell.npad
+    nt = dmats.shape[0]
+    dpat = np.any(dmats != 0, axis=0) if nt else np.zeros((n, n), bool)
+    rows = []
+    for r in range(n):
+        own = [int(c) for c in ell.col[2:int(ell.col[0, r]), r]]
+        taken = set(own) | {r, r ^ 1}
+        extra = [int(c) for c in np.nonzero(dpat[r])[0]
+                 if int(c) not in taken]
+        rows.append(own + extra)
+    slots_e = max(ell.slots, 2 + max(len(c) for c in rows))
+    col_e = np.tile(np.arange(NP, dtype=np.int32), (slots_e, 1))
+    col_e[0, :] = 2 | (2 << 8)
+    val_e = np.zeros((M, slots_e, NP))
+    val_e[:, :ell.slots] = ell.val
+    dval_e = np.zeros((nt, slots_e, NP))
+    for r in range(n):
+        col_e[1, r] = r ^ 1
+        for k, c in enumerate(rows[r]):
+            col_e[2 + k, r] = c
+        col_e[0, r] = (2 + len(rows[r])) | (int(ell.col[0, r]) << 8)
+        if nt:
+            dval_e[:, 0, r] = dmats[:, r, r]
+            if (r ^ 1) < n:
+                dval_e[:, 1, r] = dmats[:, r, r ^ 1]
+            for k, c in enumerate(rows[r]):
+                dval_e[:, 2 + k, r] = dmats[:, r, c]
+    union_t = np.any(mats != 0, axis=0).T
+    rows_t = [[int(c) for c in np.nonzero(union_t[r])[0]
+               if int(c) not in (r, r ^ 1)] for r in range(n)]
+    slots_t = 2 + max(len(c) for c in rows_t)
+    col_t = np.tile(np.arange(NP, dtype=np.int32), (slots_t, 1))
+    col_t[0, :] = 2
+    for r in range(n):
+        col_t[1, r] = r ^ 1
+        for k, c in enumerate(rows_t[r]):
+            col_t[2 + k, r] = c
+        col_t[0, r] = 2 + len(rows_t[r])
+    return col_e, val_e, col_t, dval_e, ell.slots
+
+
 class GradientProblem:
     """Batch of parametrised instances with an integrated readout."""
 
@@ -193,13 +252,39 @@ class GradientProblem:
         mats = prop._dense                                   # (M, n, n)
         mp = mats[:, perm][:, :, perm]
         dp = dconst[:, perm][:, :, perm]
-        col, val_f, val_t, dval_t = shared_adjoint_ell(prop.ell.col, mp, dp)
-        if col.shape[0] > _lib.ADJ_MAX_SLOTS:
-            raise NotImplementedError(
-                f"adjoint pattern of {col.shape[0]} slots > "
-                f"{_lib.ADJ_MAX_SLOTS}")
-        M = mats.shape[0]
         torch = _torch()
+        if prop.n > _lib.MAX_N:
+            if grad_gen >= 0:
+                raise NotImplementedError(
+                    "amplitude gradients need n <= 64")
+            col_e, val_e, col_t, dval_e, slots_f = wide_adjoint_ell(
+                prop.ell, mp, dp)
+            if max(slots_f, col_t.shape[0]) > _lib.ADJ_WIDE_MAX_SLOTS or \
+                    col_e.shape[0] > _lib.ADJ_MAX_SLOTS:
+                raise NotImplementedError(
+                    f"wide adjoint pattern {slots_f}/{col_t.shape[0]}/"
+                    f"{col_e.shape[0]} slots beyond the kernel's limits")
+            tables = dict(
+                d_col=device_tensor(col_e), d_val_f=device_tensor(val_e),
+                d_col_t=device_tensor(col_t), d_val_t=None,
+                d_dval_t=device_tensor(dval_e) if dconst.shape[0] else None,
+                slots=col_e.shape[0], slots_t=col_t.shape[0],
+                slots_f=slots_f)
+        else:
+            col, val_f, val_t, dval_t = shared_adjoint_ell(prop.ell.col, mp,
+                                                           dp)
+            if col.shape[0] > _lib.ADJ_MAX_SLOTS:
+                raise NotImplementedError(
+                    f"adjoint pattern of {col.shape[0]} slots > "
+                    f"{_lib.ADJ_MAX_SLOTS}")
+            d_col = device_tensor(col)
+            tables = dict(
+                d_col=d_col,
+                # forward values are zero beyond the forward kernel's slots
+                d_val_f=device_tensor(val_f[:, :prop.ell.slots].copy()),
+                d_col_t=d_col, d_val_t=device_tensor(val_t),
+                d_dval_t=device_tensor(dval_t) if dconst.shape[0] else None,
+                slots=prop.ell.slots, slots_t=col.shape[0], slots_f=0)
         state = dict(
             theta=theta, amplitudes=amplitudes, prop=prop, program=program,
             taylor=m, taylor_adj=ma, theta_gen=theta_gen, grad_gen=grad_gen,
@@ -208,14 +293,8 @@ class GradientProblem:
                 recs.tobytes(), dtype=np.uint8).copy()).to("cuda"),
             d_segs=device_tensor(seg_arr),
             tag_index=tag_index, tag_scale=tag_scale,
-            n_records=seg_arr.shape[0],
-            d_col=device_tensor(col),
-            # forward values are zero beyond the forward kernel's slots
-            d_val_f=device_tensor(val_f[:, :prop.ell.slots].copy()),
-            d_val_t=device_tensor(val_t),
-            d_dval_t=device_tensor(dval_t) if dconst.shape[0] else None,
-            slots_t=col.shape[0], n_inst=len(waves), dconst=dconst,
-            kernel_waves=kwaves)
+            n_records=seg_arr.shape[0], n_inst=len(waves), dconst=dconst,
+            kernel_waves=kwaves, **tables)
         return state
 
     # ------------------------------------------------------------------
@@ -247,13 +326,13 @@ class GradientProblem:
         prop = state["prop"]
         return _lib.AdjointDesc(
             n=prop.n, n_inst=state["n_inst"], n_gen=prop.K,
-            slots=prop.ell.slots, slots_t=state["slots_t"], n_exp=prop.J,
+            slots=state["slots"], slots_t=state["slots_t"], n_exp=prop.J,
             n_node=prop.S, fine=self.F, n_out=self.n_out,
             taylor=int(state["taylor"]), taylor_adj=int(state["taylor_adj"]),
             n_obs=self.obs.shape[0], flags=_lib.SSMX_COEF_WAVE, first_out=0,
             partner=1, ckpt_every=self.K, n_theta=int(state["n_theta"]),
             theta_gen=int(state["theta_gen"]),
-            grad_gen=int(state["grad_gen"]), reserved=0,
+            grad_gen=int(state["grad_gen"]), reserved=int(state["slots_f"]),
             coef_inst_stride=0, h=self.dt / self.F, t0=self.t0, dt=self.dt)
 
     def backward(self, state, seed, lambda_out=False):
@@ -276,7 +355,7 @@ class GradientProblem:
                            device="cuda")
         seed = torch.as_tensor(seed, dtype=torch.float64).to("cuda")
         seed = seed.reshape(n_inst, self.obs.shape[0]).contiguous()
-        rows = np.zeros((self.obs.shape[0], 64), dtype=_f64)
+        rows = np.zeros((self.obs.shape[0], prop.ell.npad), dtype=_f64)
         rows[:, :prop.n] = self.obs[:, prop.perm]
         d_obs = device_tensor(rows)
         n_theta = int(state["n_theta"])
@@ -289,7 +368,7 @@ class GradientProblem:
         stream = torch.cuda.current_stream().cuda_stream
         rc = lib.ssmx_adjoint(
             ctypes.byref(desc), _ptr(state["d_col"]), _ptr(state["d_val_f"]),
-            _ptr(state["d_col"]), _ptr(state["d_val_t"]),
+            _ptr(state["d_col_t"]), _ptr(state["d_val_t"]),
             _ptr(state["d_dval_t"]), _ptr(prop.d_w), _ptr(prop.d_nodes),
             _ptr(state["d_waves"]), _ptr(state["d_segs"]),
             _ptr(state["ckpt"]), _ptr(d_obs), _ptr(seed), _ptr(work),
@@ -382,4 +461,4 @@ def step_flops_adjoint(state, slots_nnz_t):
     return recompute + adj
 
 
-__all__ = ["GradientProblem", "IntegratedReadout"]
+__all__ = ["GradientProblem", "IntegratedReadout", "wide_adjoint_ell"]

@@ -23,6 +23,11 @@
   (h = 20 ps, 250,000 steps), readout of the fluorescence (every excited
   level).  n = 441 (coherent vectorisation) or 171 (compressed).
 
+* ``odmr21_gradient`` - the ``odmr21`` sweep as a GradientProblem over
+  the 7-level parameters plus the 14N quadrupole and hyperfine constants
+  (``THETA21_NAMES``), integrated fluorescence, checkpoints every 1000
+  steps.
+
 * ``odmr7_gradient`` - the ``odmr7`` sweep (n = 49) with a piecewise-
   constant microwave amplitude (200 segments of 10 ns, Rabi frequencies
   drawn N(3 MHz, 0.5 MHz), seed 3) as a ``GradientProblem`` over the
@@ -230,6 +235,42 @@ def nv7_generators(theta, use_incoherent=False):
     return np.array([g["L0"], g["Gx"], g["Gy"], g["Gz"], g["Gr"]])
 
 
+#: parameters of the 21-level gradient workload: the 7-level set plus the
+#: 14N quadrupole and hyperfine constants (Hz)
+THETA21_NAMES = THETA_NAMES + ("quadrupole", "A_par", "A_perp")
+THETA21_SCALE = np.concatenate([THETA_SCALE, [5e6, 2e6, 2.7e6]])
+
+
+def nv21_theta0():
+    """Parameter point of ``odmr21`` in ``THETA21_NAMES`` order."""
+    ground = params.nv.room.ground
+    return np.array([2.87e9, 1.42e9, 0.0, 0.0, 0.0, 65e6, 50e6,
+                     ground.thermalisation_time, ground.dephasing_time,
+                     N14_QUADRUPOLE, N14_A_PAR, N14_A_PERP])
+
+
+def nv21_generators(theta, use_incoherent=False):
+    """[L0, Gx, Gy, Gz, Gr] of the 21-level model at ``theta``."""
+    (d_gs, d_es, bx, by, bz, rel, isc, t1, t2, quad, a_par,
+     a_perp) = [float(v) for v in theta]
+    field = np.array([bx, by, bz])
+    spins, spin_int, groups = models.nv_21(
+        field, params, quadrupole=quad, hyperfine_longitudinal=a_par,
+        hyperfine_transverse=a_perp,
+        nuclear_g=N14_GAMMA / params.fundamental.nuclear_gyro)
+    spins[0][0]["D"] = math.tau * d_gs
+    spins[1][0]["D"] = math.tau * d_es
+    spins[0][0]["TS1"] = t1
+    spins[0][0]["TS2"] = t2
+    groups[((0, 0), (1, 0))] = {"rel": rel}
+    groups[((1, 0), (2, 0))] = {"s_gets_0": 8e6, "s_gets_1": isc}
+    groups[((2, 0), (0, 0))] = {"0_gets_s": 1e6, "1_gets_s": 0.7e6}
+    ops, _ = generate_atoms(spins, spin_int, groups,
+                            use_incoherent=use_incoherent)
+    g = ops["generators"]
+    return np.array([g["L0"], g["Gx"], g["Gy"], g["Gz"], g["Gr"]])
+
+
 class GradientWorkload:
     """Frozen gradient workload: problem, parameter point, loss target."""
 
@@ -285,6 +326,41 @@ def odmr7_gradient(count=1024, segments=200, seed=3, time_end=2e-6,
     name = "odmr7-gradient-n49" if not use_incoherent else \
         "odmr7-gradient-n19"
     return GradientWorkload(name, problem, nv7_theta0(), amplitudes, freqs,
+                            seed)
+
+
+def odmr21_gradient(count=512, seed=4, time_end=5e-6, time_step=2e-9,
+                    fine=100, checkpoint_every=1000, use_incoherent=False,
+                    tolerance=1e-16, adjoint_tolerance=1e-13,
+                    frequencies=None, rabi_hz=0.5e6):
+    """The ``odmr21`` sweep as a GradientProblem over ``THETA21_NAMES``
+    (the 7-level parameters and the 14N quadrupole / hyperfine constants);
+    integrated fluorescence, checkpoints every 1000 steps.  No amplitude
+    parameters (a single microwave tone per sweep point).
+    """
+    freqs = np.linspace(2.86e9, 2.88e9, count) if frequencies is None \
+        else np.asarray(frequencies, dtype=np.float64)
+    base = odmr21(1, use_incoherent=use_incoherent)
+    b1 = drive_amplitude(rabi_hz)
+
+    def waveforms(amps):
+        out = []
+        for f in freqs:
+            w = Waveform(5, base=[1.0, 0.0, 0.0, 0.0, base.excitation])
+            w.tone(1, b1, f)
+            out.append(w)
+        return out
+
+    def generators(theta):
+        return nv21_generators(theta, use_incoherent=use_incoherent)
+
+    problem = GradientProblem(
+        generators, waveforms, base.rho0, base.obs, 0.0, time_end,
+        time_step, fine, 2, checkpoint_every, tolerance=tolerance,
+        adjoint_tolerance=adjoint_tolerance, theta_scale=THETA21_SCALE)
+    name = "odmr21-gradient-n441" if not use_incoherent else \
+        "odmr21-gradient-n171"
+    return GradientWorkload(name, problem, nv21_theta0(), np.zeros(0), freqs,
                             seed)
 
 

@@ -858,8 +858,9 @@ int validate(const ssmx_adjoint_desc *d)
 {
     if (!d)
         return set_error(SSMX_EINVAL, "null descriptor");
-    if (d->n < 1 || d->n > SSMX_MAX_N)
-        return set_error(SSMX_EINVAL, "n=%d outside 1..%d", d->n, SSMX_MAX_N);
+    if (d->n < 1 || d->n > SSMX_MAX_N_WIDE)
+        return set_error(SSMX_EINVAL, "n=%d outside 1..%d", d->n,
+                         SSMX_MAX_N_WIDE);
     if (d->n_inst < 1)
         return set_error(SSMX_EINVAL, "n_inst=%d < 1", d->n_inst);
     if (d->n_gen < 1 || d->n_gen > SSMX_MAX_GEN)
@@ -920,10 +921,34 @@ int validate(const ssmx_adjoint_desc *d)
                          "segment gradients need waveform coefficients");
     if (!(d->h > 0.0) || !(d->dt > 0.0))
         return set_error(SSMX_EINVAL, "h and dt must be positive");
+    if (d->n > SSMX_MAX_N) {
+        // several wavefronts per instance (ssmx_adjoint_wide.hip)
+        if (d->partner != 1)
+            return set_error(SSMX_EINVAL, "n > 64 needs partner slots");
+        if (d->grad_gen >= 0)
+            return set_error(SSMX_EINVAL,
+                             "n > 64: segment amplitude gradients are not "
+                             "supported (grad_gen must be -1)");
+        if (d->reserved < 0 || d->reserved > d->slots)
+            return set_error(SSMX_EINVAL,
+                             "reserved (matrix slots) = %d outside 0..%d",
+                             d->reserved, d->slots);
+    } else if (d->reserved != 0) {
+        return set_error(SSMX_EINVAL, "reserved must be 0 for n <= 64");
+    }
     return SSMX_OK;
 }
 
 }  // namespace
+
+int ssmx_adjoint_wide(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
+                      const double *ell_val, const int32_t *ell_col_t,
+                      const double *dval_t, const double *cf_weights,
+                      const double *cf_nodes, const void *coef,
+                      const ssmx_segment *segments, const double *ckpt,
+                      const double *obs, const double *seed,
+                      double *workspace, double *grad_theta,
+                      double *lambda_out, hipStream_t stream);
 
 extern "C" {
 
@@ -936,6 +961,13 @@ int ssmx_adjoint_workspace_bytes(const ssmx_adjoint_desc *desc,
         return rc;
     if (!bytes)
         return set_error(SSMX_EINVAL, "null output pointer");
+    if (desc->n > SSMX_MAX_N) {
+        // inputs of every exponential of one checkpoint segment
+        const uint64_t np = (uint64_t)(desc->n + kWave - 1) / kWave * kWave;
+        *bytes = (uint64_t)desc->n_inst * (uint64_t)desc->ckpt_every *
+                 (uint64_t)desc->n_exp * np * sizeof(double);
+        return SSMX_OK;
+    }
     *bytes = (uint64_t)desc->n_inst * 2u * (uint64_t)desc->ckpt_every *
              (uint64_t)desc->n_exp * kWave * sizeof(double);
     return SSMX_OK;
@@ -955,13 +987,21 @@ int ssmx_adjoint(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
     if (rc)
         return rc;
     const bool wave_mode = (desc->flags & SSMX_COEF_WAVE) != 0;
-    if (!ell_col || !ell_val || !ell_col_t || !ell_val_t || !cf_weights ||
-        !cf_nodes || !coef || !ckpt || !obs || !seed || !workspace)
+    const bool wide = desc->n > SSMX_MAX_N;
+    if (!ell_col || !ell_val || !ell_col_t || (!ell_val_t && !wide) ||
+        !cf_weights || !cf_nodes || !coef || !ckpt || !obs || !seed ||
+        !workspace)
         return set_error(SSMX_EINVAL, "null required pointer");
     if (desc->n_theta > 0 && grad_theta && !dval_t)
         return set_error(SSMX_EINVAL, "n_theta > 0 needs dval_t");
     if (wave_mode && !segments)
         return set_error(SSMX_EINVAL, "waveform mode needs segments");
+    if (wide)
+        return ssmx_adjoint_wide(desc, ell_col, ell_val, ell_col_t, dval_t,
+                                 cf_weights, cf_nodes, coef, segments, ckpt,
+                                 obs, seed, workspace, grad_theta,
+                                 lambda_out,
+                                 static_cast<hipStream_t>(stream));
 
     AdjParams p;
     p.d = *desc;

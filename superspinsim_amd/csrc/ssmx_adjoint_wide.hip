@@ -1,0 +1,507 @@
+// ssmx_adjoint_wide.hip - adjoint of the trapezoid-integrated readout for
+// superoperators wider than one wavefront (65 <= n <= 512; gfx950), e.g.
+// the 21-level NV + 14N model (n = 441 or 171).
+//
+// Same quantities as ssmx_adjoint.hip (include/ssmx.h): per instance the
+// adjoint state lambda_0 = dLoss/d rho0 and
+//   grad_theta[t] = sum_{k,j} f_{kj,theta_gen} <Abar_kj, D_t>,
+//   Abar = sum_{i + t < MA} c_it u_i w_t^T,  c_it = i! t! / (i + t + 1)!,
+//   u_i = (A^T)^i mu / i!,  w_t = A^t y / t!,
+// for each exponential exp(A) with input y and output cotangent mu.
+//
+// One workgroup of W = ceil(n / 64) wavefronts per instance, thread r
+// owning row r (as the wide propagator).  Per checkpoint segment, last
+// first:
+//  1. recomputation from the checkpoint with the propagator's own
+//     exponential (ssmx_wide_exp.h, bit-identical states), storing the
+//     input of every exponential in the workspace;
+//  2. the reverse sweep: per exponential (last applied first) the chain
+//     u_i on the transposed pattern (lambda <- exp(A)^T lambda = sum u_i,
+//     and v_t = sum_i c_it u_i, row-local), then the chain w_t on the
+//     forward pattern extended by the derivative tables' non-zeros, where
+//     each exchange of w_t also adds f_theta v_t[r] w_t[c] to the row's
+//     accumulator B[r][c]; finally lambda += w_k s (readout weight).
+// At the end grad_theta[t] = sum_{r,c} B[r][c] D_t[r][c] (one reduction per
+// parameter).  Only the matrices' own slots are staged in LDS; the values
+// of A^T are read from them through per-thread source offsets, found once
+// at launch by scanning row c of the forward pattern for column r.
+//
+// No counterpart in the reference (it has no gradients); it differentiates
+// the path ssmx_propagate replaces (generate_simulator.py:1945-2116).
+
+#include "ssmx_wide_exp.h"
+
+namespace {
+
+using ssmx::node_time;
+using ssmx::set_error;
+using ssmx::wave_coef_seek;
+using ssmx::wave_max;
+using ssmx::wave_sum;
+
+// one instance per workgroup: at most 512 rows = 8 wavefronts
+constexpr int kMaxWaves = SSMX_MAX_N_WIDE / kWave;
+
+struct AWParams {
+    ssmx_adjoint_desc d;
+    int slots_f;            // staged (matrix) slots
+    const int32_t *col;     // [slots][NP] extended forward pattern
+    const double *val;      // [M][slots][NP]
+    const int32_t *col_t;   // [slots_t][NP] transposed pattern
+    const double *dval;     // [n_theta][slots][NP]
+    const double *w;
+    const double *nodes;
+    const double *coef;
+    const ssmx_wave *waves;
+    const ssmx_segment *segs;
+    const double *ckpt;
+    const double *obs;
+    const double *seed;
+    double *work;
+    double *grad_theta;
+    double *lambda_out;
+};
+
+// i! t! / (i + t + 1)! = 1 / ((i + t + 1) binom(i + t, i))
+__host__ __device__ constexpr double frechet_coef(int i, int t)
+{
+    double b = 1.0;
+    for (int k = 1; k <= i; ++k)
+        b = b * (double)(t + k) / (double)k;
+    return 1.0 / ((double)(i + t + 1) * b);
+}
+
+// h sum_s w[j,s] c_m(t_s) of propagator step `step` in the propagator's
+// rounding order (non-rotating frame), for steps visited in any order.
+__device__ double adj_factor(const AWParams &p, const double *coef_i,
+                             const ssmx_wave *wv, long step, int j, int m,
+                             int &cursor)
+{
+    const ssmx_adjoint_desc &d = p.d;
+    const int S = d.n_node, K = d.n_gen;
+    const long i_loc = step / d.fine;
+    const int r = (int)(step - i_loc * d.fine);
+    const long ig = (long)d.first_out + i_loc;
+    double cw = 0.0;
+    for (int s = 0; s < S; ++s) {
+        double c;
+        if (wv) {
+            int rec;
+            double damp;
+            c = wave_coef_seek(
+                wv, p.segs, m,
+                node_time(d.t0, d.dt, d.fine, r, ig, p.nodes[s]), cursor,
+                rec, damp);
+        } else {
+            c = coef_i[((size_t)(i_loc * d.fine + r) * S + s) * K + m];
+        }
+        cw = fma(p.w[j * S + s], c, cw);
+    }
+    return d.h * cw;
+}
+
+// Workspace access bypasses the vector L1 (agent-scope relaxed atomics are
+// plain loads and stores with coherence bits), so a thread always reads
+// back its own latest store of a slot that earlier segments also used.
+__device__ __forceinline__ void ws_store(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double ws_load(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One exchange of w (buffer P) in the reverse sweep: B += cv * w[c] over
+// the extended pattern; returns (A w) for this row over the SF staged
+// slots (a = 0 beyond the matrices' own entries).
+template <int SF, int SE, int P, bool PRODUCT>
+__device__ __forceinline__ double exchange_accumulate(
+    double w, double cv, double (&B)[SE], const double (&a)[SF],
+    double *own, const double *const (&gpe)[SE])
+{
+    own[P * kBuf] = w;
+    __syncthreads();
+    double g[SE];
+    g[0] = w;
+    g[1] = swap_pair(w);
+#pragma unroll
+    for (int sl = 2; sl < SE; ++sl)
+        g[sl] = gpe[sl][P * kBuf];
+#pragma unroll
+    for (int sl = 0; sl < SE; ++sl)
+        B[sl] = fma(cv, g[sl], B[sl]);
+    if (!PRODUCT)
+        return 0.0;
+    double y0 = a[0] * g[0];
+    double y1 = a[1] * g[1];
+#pragma unroll
+    for (int sl = 2; sl < SF; ++sl) {
+        if (sl & 1)
+            y1 = fma(a[sl], g[sl], y1);
+        else
+            y0 = fma(a[sl], g[sl], y0);
+    }
+    return y0 + y1;
+}
+
+// Reverse of one exponential with input y; enters and leaves with buffer
+// 0 next (2 MA exchanges).  lam: output cotangent in, input cotangent out.
+template <int SF, int SE, int MA>
+__device__ __forceinline__ void reverse_exp(
+    double y, double &lam, double (&B)[SE], const ExpArgs &e, int row,
+    double factor, double fth, double inv_lane, double *own,
+    const double *const (&gpe)[SE], const double *const (&gpt)[SF],
+    const int (&src)[SF])
+{
+    static_assert(MA % 2 == 0, "exchanges are paired");
+    // row of A^T from the staged values of A
+    double at[SF];
+#pragma unroll
+    for (int sl = 0; sl < SF; ++sl)
+        at[sl] = 0.0;
+    for (int m = 0; m < e.M; ++m) {
+        const double f = read_lane(factor, e.base + m);
+        const double *plane = e.lv + (size_t)m * SF * e.NP;
+        double vv[SF];
+#pragma unroll
+        for (int sl = 0; sl < SF; ++sl)
+            vv[sl] = plane[src[sl] >= 0 ? src[sl] : 0];
+#pragma unroll
+        for (int sl = 0; sl < SF; ++sl)
+            at[sl] = fma(f, src[sl] >= 0 ? vv[sl] : 0.0, at[sl]);
+    }
+    // u chain (transposed products), lambda and v_t
+    double u = lam;
+    double acc = lam;
+    double v[MA];
+#pragma unroll
+    for (int t = 0; t < MA; ++t)
+        v[t] = frechet_coef(0, t) * u;
+#pragma unroll
+    for (int i = 1; i <= MA; i += 2) {
+        u = wide_product<SF, SF, 0>(at, u, own, gpt, true) *
+            read_lane(inv_lane, i);
+        acc += u;
+#pragma unroll
+        for (int t = 0; t < MA; ++t)
+            if (t < MA - i)
+                v[t] = fma(frechet_coef(i, t), u, v[t]);
+        u = wide_product<SF, SF, 1>(at, u, own, gpt, true) *
+            read_lane(inv_lane, i + 1);
+        acc += u;
+#pragma unroll
+        for (int t = 0; t < MA; ++t)
+            if (t < MA - i - 1)
+                v[t] = fma(frechet_coef(i + 1, t), u, v[t]);
+    }
+    lam = acc;
+    // row of A
+    double a[SF];
+#pragma unroll
+    for (int sl = 0; sl < SF; ++sl)
+        a[sl] = 0.0;
+    for (int m = 0; m < e.M; ++m) {
+        const double f = read_lane(factor, e.base + m);
+        const double2 *pv =
+            reinterpret_cast<const double2 *>(e.lv + (size_t)m * SF * e.NP) +
+            row;
+        double2 vv[SF / 2];
+#pragma unroll
+        for (int pr = 0; pr < SF / 2; ++pr)
+            vv[pr] = pv[pr * e.NP];
+#pragma unroll
+        for (int pr = 0; pr < SF / 2; ++pr) {
+            a[2 * pr] = fma(f, vv[pr].x, a[2 * pr]);
+            a[2 * pr + 1] = fma(f, vv[pr].y, a[2 * pr + 1]);
+        }
+    }
+    // w chain with the accumulation of f_theta Abar
+    double w = y;
+#pragma unroll
+    for (int t = 0; t < MA; t += 2) {
+        w = exchange_accumulate<SF, SE, 0, true>(w, fth * v[t], B, a, own,
+                                                 gpe) *
+            read_lane(inv_lane, t + 1);
+        if (t + 2 < MA)
+            w = exchange_accumulate<SF, SE, 1, true>(w, fth * v[t + 1], B,
+                                                     a, own, gpe) *
+                read_lane(inv_lane, t + 2);
+        else
+            exchange_accumulate<SF, SE, 1, false>(w, fth * v[t + 1], B, a,
+                                                  own, gpe);
+    }
+}
+
+template <int SF, int SE, int MA>
+__global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
+    AWParams p, int W)
+{
+    extern __shared__ double lds[];
+    const ssmx_adjoint_desc &d = p.d;
+    const int NP = W * kWave;
+    const int row = threadIdx.x;
+    const int lane = row & (kWave - 1);
+    const int wave = row / kWave;
+    const int inst = blockIdx.x;
+    const int n = d.n;
+    const int M = d.n_gen;
+    const int J = d.n_exp;
+    const int slots = d.slots;
+    const bool wave_mode = (d.flags & SSMX_COEF_WAVE) != 0;
+
+    // LDS: staged matrix values [m][slot / 2][row][slot % 2] (SF slots),
+    // two exchange buffers kBuf apart, one partial sum per wavefront
+    double *lv = lds;
+    double *xb0 = lv + (size_t)M * SF * NP;
+    double *red = xb0 + 2 * kBuf;
+    const int staged = p.slots_f < SF ? p.slots_f : SF;
+    for (int e = row; e < M * SF * NP; e += blockDim.x) {
+        const int half = e & 1;
+        const int r = (e >> 1) % NP;
+        const int pair = (e / (2 * NP)) % (SF / 2);
+        const int m = e / (NP * SF);
+        const int slot = 2 * pair + half;
+        lv[e] = slot < staged ? p.val[((size_t)m * slots + slot) * NP + r]
+                              : 0.0;
+    }
+    __syncthreads();
+
+    // extended forward pattern: gathers of the reverse sweep; its first SF
+    // slots also serve the recomputation.  Slot 0's entry holds the row's
+    // used slots (bits 0-7) and, if set, its matrix slots (bits 8-15).
+    const int head = p.col[row];
+    const int cnt_e = head & 0xFF;
+    const int cnt_f = (head >> 8) & 0xFF;
+    const double *gpe[SE];
+    const double *gpf[SF];
+#pragma unroll
+    for (int sl = 0; sl < SE; ++sl) {
+        const int c = sl == 0 ? row
+                              : (sl < slots ? p.col[sl * NP + row] : row);
+        gpe[sl] = xb0 + (c & (kBuf - 1));
+    }
+#pragma unroll
+    for (int sl = 0; sl < SF; ++sl)
+        gpf[sl] = gpe[sl];
+    // recomputation slot class: the wavefront's widest row of the matrices,
+    // capped at the staged slots (columns beyond carry derivative entries
+    // only)
+    int ws_f = wave_max(cnt_f > 0 ? cnt_f : cnt_e);
+    ws_f = ws_f < staged ? ws_f : staged;
+
+    // transposed pattern: gathers, and where each A^T value lives among the
+    // staged values (row c, slot s' of A with column r)
+    const int cnt_t = p.col_t[row];
+    const double *gpt[SF];
+    int src[SF];
+#pragma unroll
+    for (int sl = 0; sl < SF; ++sl) {
+        int c = row;
+        int s = -1;
+        if (sl == 0) {
+            s = 0;
+        } else if (sl < d.slots_t && sl < cnt_t) {
+            c = p.col_t[sl * NP + row];
+            if (c < 0 || c >= NP)
+                c = row;
+            else if (c == (row ^ 1))
+                s = 1;
+            else
+                for (int k = 2; k < staged; ++k)
+                    if (p.col[k * NP + c] == row) {
+                        s = k;
+                        break;
+                    }
+        }
+        gpt[sl] = xb0 + (c & (kBuf - 1));
+        src[sl] = s < 0 ? -1 : (s >> 1) * 2 * NP + c * 2 + (s & 1);
+    }
+
+    double srow = 0.0;
+    for (int o = 0; o < d.n_obs; ++o)
+        srow = fma(p.seed[(size_t)inst * d.n_obs + o], p.obs[o * NP + row],
+                   srow);
+    const double inv_lane = kInvW[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+
+    double B[SE];
+#pragma unroll
+    for (int sl = 0; sl < SE; ++sl)
+        B[sl] = 0.0;
+    double lam = 0.5 * d.h * srow;
+
+    const long T = (long)d.n_out * d.fine;
+    const int K = d.ckpt_every;
+    const long P = (T + K - 1) / K;
+    const int JM = J * M;
+    const int my_j = lane / M;
+    const int my_m = lane % M;
+    int cursor = 0;
+    const double *coef_i =
+        wave_mode ? nullptr : p.coef + (size_t)inst * d.coef_inst_stride;
+    const ssmx_wave *wv = wave_mode ? p.waves + inst : nullptr;
+    const int tg = d.theta_gen;
+    ExpArgs e;
+    e.lv = lv;
+    e.NP = NP;
+    e.M = M;
+    e.base = 0;
+    e.substeps = 1;
+    e.taylor = d.taylor;
+    e.inv_sub = 1.0;
+    e.partner = true;
+    int parity = 0;
+    double *own = xb0 + row;
+    double *ws = p.work + (size_t)inst * K * J * NP;
+
+    for (long seg = P - 1; seg >= 0; --seg) {
+        const long k0 = seg * K;
+        const long k1 = T < k0 + K ? T : k0 + K;
+        // 1. recomputation from the checkpoint
+        double x = row < n ? p.ckpt[((size_t)inst * P + seg) * n + row]
+                           : 0.0;
+        for (long k = k0; k < k1; ++k) {
+            const double factor =
+                lane < JM ? adj_factor(p, coef_i, wv, k, my_j, my_m, cursor)
+                          : 0.0;
+            for (int j = J - 1; j >= 0; --j) {
+                ws_store(ws + ((size_t)(k - k0) * J + j) * NP + row, x);
+                e.base = j * M;
+                x = wide_exp_dispatch<SF>(ws_f, x, e, row, factor, inv_lane,
+                                          own, gpf, parity);
+            }
+        }
+        if (parity) {
+            // the reverse sweep starts on buffer 0: finish every read of it
+            __syncthreads();
+            parity = 0;
+        }
+        // 2. reverse sweep
+        for (long k = k1 - 1; k >= k0; --k) {
+            const double factor =
+                lane < JM ? adj_factor(p, coef_i, wv, k, my_j, my_m, cursor)
+                          : 0.0;
+            for (int j = 0; j < J; ++j) {
+                const double y =
+                    ws_load(ws + ((size_t)(k - k0) * J + j) * NP + row);
+                e.base = j * M;
+                const double fth =
+                    tg >= 0 ? read_lane(factor, j * M + tg) : 0.0;
+                reverse_exp<SF, SE, MA>(y, lam, B, e, row, factor, fth,
+                                        inv_lane, own, gpe, gpt, src);
+            }
+            lam = fma(k == 0 ? 0.5 * d.h : d.h, srow, lam);
+        }
+    }
+
+    if (p.lambda_out && row < n)
+        p.lambda_out[(size_t)inst * n + row] = lam;
+    if (p.grad_theta) {
+        for (int t = 0; t < d.n_theta; ++t) {
+            double part = 0.0;
+#pragma unroll
+            for (int sl = 0; sl < SE; ++sl)
+                if (sl < slots)
+                    part = fma(B[sl],
+                               p.dval[((size_t)t * slots + sl) * NP + row],
+                               part);
+            part = wave_sum(part);
+            if (lane == 0)
+                red[wave] = part;
+            __syncthreads();
+            if (row == 0) {
+                double s = 0.0;
+                for (int k = 0; k < W; ++k)
+                    s += red[k];
+                p.grad_theta[(size_t)inst * d.n_theta + t] = s;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int SF, int SE, int MA>
+int launch(const AWParams &p, hipStream_t stream)
+{
+    const int W = (p.d.n + kWave - 1) / kWave;
+    const int NP = W * kWave;
+    const size_t lds =
+        ((size_t)p.d.n_gen * SF * NP + 2 * kBuf + W) * sizeof(double);
+    if (lds > 160 * 1024)
+        return set_error(SSMX_EINVAL,
+                         "n=%d: %d matrices x %d staged slots need %zu "
+                         "bytes of LDS (> 160 KiB)",
+                         p.d.n, p.d.n_gen, SF, lds);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&adjoint_wide_kernel<SF, SE, MA>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
+                             hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL((adjoint_wide_kernel<SF, SE, MA>), dim3(p.d.n_inst),
+                       dim3(NP), lds, stream, p, W);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(SSMX_ELAUNCH, "wide adjoint launch: %s",
+                         hipGetErrorString(e));
+    return SSMX_OK;
+}
+
+template <int SF, int SE>
+int pick_degree(const AWParams &p, hipStream_t s)
+{
+    if (p.d.taylor_adj <= 8)
+        return launch<SF, SE, 8>(p, s);
+    return launch<SF, SE, 10>(p, s);
+}
+
+}  // namespace
+
+// Entry from ssmx_adjoint (ssmx_adjoint.hip) for n > 64; the common
+// descriptor checks and pointers were validated there.
+int ssmx_adjoint_wide(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
+                      const double *ell_val, const int32_t *ell_col_t,
+                      const double *dval_t, const double *cf_weights,
+                      const double *cf_nodes, const void *coef,
+                      const ssmx_segment *segments, const double *ckpt,
+                      const double *obs, const double *seed,
+                      double *workspace, double *grad_theta,
+                      double *lambda_out, hipStream_t stream)
+{
+    const bool wave_mode = (desc->flags & SSMX_COEF_WAVE) != 0;
+    AWParams p;
+    p.d = *desc;
+    p.slots_f = desc->reserved > 0 ? desc->reserved : desc->slots;
+    p.col = ell_col;
+    p.val = ell_val;
+    p.col_t = ell_col_t;
+    p.dval = dval_t;
+    p.w = cf_weights;
+    p.nodes = cf_nodes;
+    p.coef = wave_mode ? nullptr : static_cast<const double *>(coef);
+    p.waves = wave_mode ? static_cast<const ssmx_wave *>(coef) : nullptr;
+    p.segs = segments;
+    p.ckpt = ckpt;
+    p.obs = obs;
+    p.seed = seed;
+    p.work = workspace;
+    p.grad_theta = desc->n_theta > 0 ? grad_theta : nullptr;
+    p.lambda_out = lambda_out;
+    const int sf = p.slots_f > desc->slots_t ? p.slots_f : desc->slots_t;
+    const int se = desc->slots;
+    if (sf <= 8 && se <= 16)
+        return pick_degree<8, 16>(p, stream);
+    if (sf <= 12 && se <= 20)
+        return pick_degree<12, 20>(p, stream);
+    if (sf <= 16 && se <= 24)
+        return pick_degree<16, 24>(p, stream);
+    if (sf <= 16 && se <= 32)
+        return pick_degree<16, 32>(p, stream);
+    return set_error(SSMX_EINVAL,
+                     "n > 64 adjoint supports at most 16 matrix / transposed "
+                     "slots and 32 extended slots (got %d / %d / %d)",
+                     p.slots_f, desc->slots_t, desc->slots);
+}

@@ -1,0 +1,169 @@
+// ssmx_wide_exp.h - the exponential x <- exp(A) x of one row per thread,
+// shared by the wide propagator (ssmx_wide.hip) and the wide adjoint's
+// recomputation (ssmx_adjoint_wide.hip), so that both give the same states
+// bit for bit.
+//
+// Several wavefronts hold one instance, thread r owning row r.  Each Taylor
+// term is written to an LDS exchange buffer of the instance and combined
+// over the row's ELL slots after one workgroup barrier: slot 0 is the
+// diagonal, slot 1 the partner r ^ 1 (DPP), other slots are gathered.  Two
+// buffers alternate, kBuf doubles apart, so the gather addresses are fixed
+// and the buffer is an immediate offset.  Generator values are staged in
+// LDS as [m][slot / 2][row][slot % 2].
+#pragma once
+
+#include "ssmx_common.h"
+
+namespace {
+
+using ssmx::kWave;
+using ssmx::read_lane;
+using ssmx::swap_pair;
+
+// largest workgroup: 16 wavefronts (1024 threads)
+constexpr int kMaxBlockWaves = 16;
+// stride between an instance's two exchange buffers (doubles): one buffer
+// holds up to 512 rows; 4 KiB fits a ds_read immediate offset
+constexpr int kBuf = SSMX_MAX_N_WIDE;
+
+__constant__ double kInvW[SSMX_MAX_TAYLOR + 1] = {
+    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
+    1.0 / 6,    1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
+    1.0 / 12,   1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
+    1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
+    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
+    1.0 / 30,   1.0 / 31,   1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35,
+    1.0 / 36,   1.0 / 37,   1.0 / 38, 1.0 / 39, 1.0 / 40};
+
+// Wave-uniform inputs of one exponential.
+struct ExpArgs {
+    const double *lv;       // staged values [m][slot / 2][row][slot % 2]
+    int NP, M, base;        // rows, matrices, first factor lane
+    int substeps, taylor;
+    double inv_sub;
+    bool partner;
+};
+
+// y = A v for this thread's row, reading buffer P (0 or 1): v is written to
+// the row's own entry, then WS slots are combined (slot 0 diagonal, slot 1
+// partner, others gathered).  Two accumulation chains (even and odd
+// slots), as in ssmx_propagate.hip.
+template <int SLOTS, int WS, int P>
+__device__ __forceinline__ double wide_product(
+    const double (&a)[WS], double v, double *own,
+    const double *const (&gp)[SLOTS], bool partner)
+{
+    own[P * kBuf] = v;
+    __syncthreads();
+    double g[WS];
+    g[0] = v;
+    g[1] = partner ? swap_pair(v) : gp[1][P * kBuf];
+#pragma unroll
+    for (int sl = 2; sl < WS; ++sl)
+        g[sl] = gp[sl][P * kBuf];
+    double y0 = a[0] * g[0];
+    double y1 = a[1] * g[1];
+#pragma unroll
+    for (int sl = 2; sl < WS; ++sl) {
+        if (sl & 1)
+            y1 = fma(a[sl], g[sl], y1);
+        else
+            y0 = fma(a[sl], g[sl], y0);
+    }
+    return y0 + y1;
+}
+
+// x <- exp(A_j) x for this thread's row: assemble the row of A_j from the
+// staged values (slots < WS), then s substeps of the degree-m Taylor
+// action.  `parity` is the buffer the next product writes.
+template <int SLOTS, int WS>
+__device__ __forceinline__ double wide_exp(double x, const ExpArgs &e,
+                                           int row, double factor,
+                                           double inv_lane, double *own,
+                                           const double *const (&gp)[SLOTS],
+                                           int &parity)
+{
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    double a[WS];
+#pragma unroll
+    for (int sl = 0; sl < WS; ++sl)
+        a[sl] = 0.0;
+    for (int m = 0; m < e.M; ++m) {
+        const double f = read_lane(factor, e.base + m);
+        const double2 *v =
+            reinterpret_cast<const double2 *>(e.lv +
+                                              (size_t)m * SLOTP * e.NP) +
+            row;
+        double2 vv[WS / 2];
+#pragma unroll
+        for (int pr = 0; pr < WS / 2; ++pr)
+            vv[pr] = v[pr * e.NP];
+#pragma unroll
+        for (int pr = 0; pr < WS / 2; ++pr) {
+            a[2 * pr] = fma(f, vv[pr].x, a[2 * pr]);
+            a[2 * pr + 1] = fma(f, vv[pr].y, a[2 * pr + 1]);
+        }
+    }
+    if (e.substeps > 1) {
+#pragma unroll
+        for (int sl = 0; sl < WS; ++sl)
+            a[sl] *= e.inv_sub;
+    }
+    // term_t = (A term_{t-1}) / t; acc += term_t
+    auto taylor_term = [&](double y, int t, double &term, double &acc) {
+#pragma clang fp contract(off)
+        term = y * read_lane(inv_lane, t);
+        acc += term;
+    };
+    for (int sub = 0; sub < e.substeps; ++sub) {
+        double term = x;
+        double acc = x;
+        // products alternate buffers; the loop body is one pair (buffer 0,
+        // then 1), so both buffer offsets are immediates
+        int t = 1;
+        if (parity && t <= e.taylor) {
+            taylor_term(wide_product<SLOTS, WS, 1>(a, term, own, gp,
+                                                   e.partner),
+                        t, term, acc);
+            ++t;
+            parity = 0;
+        }
+        for (; t + 1 <= e.taylor; t += 2) {
+            taylor_term(wide_product<SLOTS, WS, 0>(a, term, own, gp,
+                                                   e.partner),
+                        t, term, acc);
+            taylor_term(wide_product<SLOTS, WS, 1>(a, term, own, gp,
+                                                   e.partner),
+                        t + 1, term, acc);
+        }
+        if (t <= e.taylor) {
+            taylor_term(wide_product<SLOTS, WS, 0>(a, term, own, gp,
+                                                   e.partner),
+                        t, term, acc);
+            parity = 1;
+        }
+        x = acc;
+    }
+    return x;
+}
+
+// wide_exp for the smallest even slot class WS >= ws (wave-uniform).
+template <int SLOTS, int WS = 2>
+__device__ __forceinline__ double wide_exp_dispatch(
+    int ws, double x, const ExpArgs &e, int row, double factor,
+    double inv_lane, double *own, const double *const (&gp)[SLOTS],
+    int &parity)
+{
+    if constexpr (WS >= SLOTS) {
+        return wide_exp<SLOTS, SLOTS>(x, e, row, factor, inv_lane, own, gp,
+                                      parity);
+    } else {
+        if (ws <= WS)
+            return wide_exp<SLOTS, WS>(x, e, row, factor, inv_lane, own, gp,
+                                       parity);
+        return wide_exp_dispatch<SLOTS, WS + 2>(ws, x, e, row, factor,
+                                                inv_lane, own, gp, parity);
+    }
+}
+
+}  // namespace

@@ -133,10 +133,19 @@ def test_adjoint_workspace_size(lib):
     assert lib.ssmx_adjoint_workspace_bytes(ctypes.byref(d),
                                             ctypes.byref(out)) == 0
     assert out.value == 3 * 2 * 7 * 2 * 64 * 8
+    # n > 64: the inputs of every exponential of one segment, stride NP
+    d = _adesc(n=441, n_inst=3, ckpt_every=7, n_exp=2, grad_gen=-1,
+               reserved=2)
+    assert lib.ssmx_adjoint_workspace_bytes(ctypes.byref(d),
+                                            ctypes.byref(out)) == 0
+    assert out.value == 3 * 7 * 2 * 448 * 8
 
 
 @pytest.mark.parametrize("bad", [
-    dict(n=65), dict(n_inst=0), dict(slots=33), dict(slots_t=0),
+    dict(n=513), dict(n=100, grad_gen=1), dict(n=100, grad_gen=-1,
+                                                  partner=0),
+    dict(n=100, grad_gen=-1, reserved=3), dict(reserved=1),
+    dict(n_inst=0), dict(slots=33), dict(slots_t=0),
     dict(taylor_adj=11), dict(taylor_adj=0), dict(ckpt_every=0),
     dict(n_theta=17), dict(theta_gen=2), dict(theta_gen=-1),
     dict(grad_gen=2), dict(flags=_lib.SSMX_ROTATING),

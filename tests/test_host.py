@@ -98,6 +98,52 @@ def test_wide_propagator_orders_pairs_by_width():
     assert sum(per_wave) <= 64
 
 
+def test_wide_adjoint_tables():
+    """Extended forward, transposed and derivative tables for n > 64."""
+    from superspinsim_amd.adjoint import wide_adjoint_ell
+    rng = np.random.default_rng(7)
+    n, M, nt = 130, 2, 3
+    mats = np.where(rng.random((M, n, n)) < 0.04,
+                    rng.standard_normal((M, n, n)), 0.0)
+    mats[:, np.arange(n), np.arange(n)] = 1.0
+    pat = np.any(mats != 0, axis=0)
+    perm = pair_permutation(pat)
+    perm = perm[width_order(pat[perm][:, perm])]
+    mp = mats[:, perm][:, :, perm]
+    dmats = np.where(rng.random((nt, n, n)) < 0.03,
+                     rng.standard_normal((nt, n, n)), 0.0)
+    ell = EllStack(mp)
+    col_e, val_e, col_t, dval_e, slots_f = wide_adjoint_ell(ell, mp, dmats)
+    NP = ell.npad
+    assert slots_f == ell.slots and col_e.shape[1] == NP
+    used = col_e[0] & 0xFF
+    assert np.array_equal(col_e[0] >> 8, ell.col[0])
+    # the matrices' columns first, in the propagator's order and values
+    assert np.array_equal(col_e[1:ell.slots][:, :n][ell.col[1:, :n] != 
+                          np.arange(n)], ell.col[1:, :n][ell.col[1:, :n] !=
+                          np.arange(n)])
+    assert np.array_equal(val_e[:, :ell.slots], ell.val)
+    assert not np.any(val_e[:, ell.slots:])
+    # D is exactly representable on the extended pattern
+    D = np.zeros((nt, n, n))
+    for r in range(n):
+        for sl in range(int(used[r])):
+            c = r if sl == 0 else int(col_e[sl, r])
+            if c < n:
+                D[:, r, c] += dval_e[:, sl, r]
+    assert np.array_equal(D, dmats)
+    # transposed pattern covers A^T; every entry has a staged source that
+    # the kernel's scan of row c finds (slot 0 diagonal, 1 partner, else
+    # the slot of column r among the matrices' slots)
+    for r in range(n):
+        cols = {r, r ^ 1} | {int(c) for c in col_t[2:col_t[0, r], r]}
+        assert set(np.nonzero(pat[perm][:, perm][:, r])[0]) <= cols
+        for sl in range(2, int(col_t[0, r])):
+            c = int(col_t[sl, r])
+            hits = [k for k in range(2, slots_f) if ell.col[k, c] == r]
+            assert len(hits) == 1
+
+
 @pytest.mark.parametrize("norm", [0.01, 0.19, 0.8, 3.0, 40.0])
 def test_taylor_plan_meets_tolerance(norm):
     m, s = taylor_plan(norm, tol=1e-16)
