@@ -21,7 +21,10 @@ HIP graph) and reports propagator steps/s over the whole batch.
 The hyperfine leg runs the 21-level NV + 14N sweep (512 points, 250,000
 steps each, fluorescence every 2 ns) at n = 441 and at n = 171 through the
 wide propagator (several wavefronts per instance), with a bounded CPU sample
-of the reference algorithm at n = 441.
+of the reference algorithm at n = 441.  The hyperfine gradient leg runs the
+n = 441 sweep forward with checkpoints and then the adjoint, for gradients
+of the integrated fluorescence over 12 parameters (the 7-level set and the
+14N quadrupole and hyperfine constants), and reports the same cost ratio.
 
 Prints one JSON line (rank 0, N = 1 GPU).
 """
@@ -85,6 +88,12 @@ def parse():
                    help="truncate the hyperfine sweep to this many output "
                         "intervals (profiling runs)")
     p.add_argument("--hyperfine-cpu-steps", type=int, default=16)
+    p.add_argument("--no-hyperfine-adjoint", action="store_true",
+                   help="skip the 21-level gradient (adjoint) leg")
+    p.add_argument("--hyperfine-adjoint-only", action="store_true")
+    p.add_argument("--hyperfine-adjoint-instances", type=int, default=512)
+    p.add_argument("--hyperfine-adjoint-time", type=float, default=5e-6,
+                   help="simulated time of the gradient sweep (s)")
     return p.parse_args()
 
 
@@ -359,6 +368,61 @@ def hyperfine_leg(args):
     return legs
 
 
+def hyperfine_adjoint_leg(args):
+    """Forward + adjoint of the 21-level sweep (n = 441): gradients of the
+    integrated fluorescence over the 7-level and 14N parameters."""
+    import numpy as np
+    import torch
+    from superspinsim_amd import configs
+    wl = configs.odmr21_gradient(count=args.hyperfine_adjoint_instances,
+                                 time_end=args.hyperfine_adjoint_time)
+    pb = wl.problem
+    t_setup = time.perf_counter()
+    state = pb.setup(wl.theta, wl.amplitudes)
+    plain_state = pb.setup(wl.theta, wl.amplitudes, derivatives=False)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    steps = pb.n_out * pb.F
+    print(f"hyperfine gradient workload {wl.name}: {state['n_inst']} x "
+          f"{steps} steps, {len(configs.THETA21_NAMES)} parameters, setup "
+          f"{setup_s:.1f} s", file=sys.stderr, flush=True)
+    pb.forward(wl.theta, wl.amplitudes, plain_state)      # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pb.forward(wl.theta, wl.amplitudes, plain_state)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    out, st = pb.forward(wl.theta, wl.amplitudes, state)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    res = pb.backward(st, torch.ones_like(out))
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    g = res["theta"].cpu().numpy()
+    fwd, fwd_ckpt, bwd = t1 - t0, t2 - t1, t3 - t2
+    print(f"  forward {fwd:.3f} s, with checkpoints {fwd_ckpt:.3f} s, "
+          f"backward {bwd:.3f} s", file=sys.stderr, flush=True)
+    return {
+        "workload": wl.name,
+        "instances": state["n_inst"],
+        "steps_per_instance": steps,
+        "parameters": list(configs.THETA21_NAMES),
+        "checkpoint_every": pb.K,
+        "taylor_degree": int(state["taylor"]),
+        "adjoint_degree": int(state["taylor_adj"]),
+        "slots_matrix_transposed_extended": [int(state["slots_f"]),
+                                             int(state["slots_t"]),
+                                             int(state["slots"])],
+        "setup_s": setup_s,
+        "forward_s": fwd,
+        "forward_ckpt_s": fwd_ckpt,
+        "backward_s": bwd,
+        "ratio": (fwd_ckpt + bwd) / fwd,
+        "grad_finite": bool(np.all(np.isfinite(g))),
+        "grad_theta_sum": [float(v) for v in g.sum(axis=0)],
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -387,6 +451,9 @@ def main():
         return
     if args.hyperfine_only:
         print(json.dumps({"hyperfine": hyperfine_leg(args)}))
+        return
+    if args.hyperfine_adjoint_only:
+        print(json.dumps({"hyperfine_adjoint": hyperfine_adjoint_leg(args)}))
         return
 
     wl = workload(args.config, args.instances)
@@ -503,6 +570,8 @@ def main():
         result["protocols"] = protocol_leg(args)
     if not args.no_hyperfine and args.config == "odmr7":
         result["hyperfine"] = hyperfine_leg(args)
+    if not args.no_hyperfine_adjoint and args.config == "odmr7":
+        result["hyperfine_adjoint"] = hyperfine_adjoint_leg(args)
     if not args.no_cpu:
         outputs = args.cpu_outputs or (128 if wl.n > 20 else 999)
         result["cpu_baseline"] = cpu_baseline(wl, outputs)

@@ -45,6 +45,8 @@ constexpr int kMaxWaves = SSMX_MAX_N_WIDE / kWave;
 struct AWParams {
     ssmx_adjoint_desc d;
     int slots_f;            // staged (matrix) slots
+    int zero_entry;         // offset of a staged value that is 0 in every
+                            // plane
     const int32_t *col;     // [slots][NP] extended forward pattern
     const double *val;      // [M][slots][NP]
     const int32_t *col_t;   // [slots_t][NP] transposed pattern
@@ -113,124 +115,169 @@ __device__ __forceinline__ double ws_load(const double *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One exchange of w (buffer P) in the reverse sweep: B += cv * w[c] over
-// the extended pattern; returns (A w) for this row over the SF staged
-// slots (a = 0 beyond the matrices' own entries).
-template <int SF, int SE, int P, bool PRODUCT>
-__device__ __forceinline__ double exchange_accumulate(
-    double w, double cv, double (&B)[SE], const double (&a)[SF],
-    double *own, const double *const (&gpe)[SE])
+// Wave-uniform inputs of one reverse exponential.
+struct RevArgs {
+    const double *lv;   // staged matrix values [m][slot / 2][row][slot % 2]
+    int NP, M, base;    // rows, matrices, first factor lane
+    int row;
+    double factor;      // this lane's factor (lane j M + m)
+    double fth;         // factor of the matrix the derivative tables follow
+    double inv_lane;    // 1 / lane
+    double *own;        // the row's entry in exchange buffer 0
+};
+
+// Step I (1..MA) of the transposed chain on buffer (I - 1) & 1:
+// u <- (A^T u) / I, lambda += u, v_t += c_It u for t < MA - I.
+template <int SF, int WT, int MA, int I>
+__device__ __forceinline__ void u_chain(double &u, double &acc,
+                                        double (&v)[MA],
+                                        const double (&at)[WT],
+                                        const RevArgs &r,
+                                        const double *const (&gpt)[SF])
 {
-    own[P * kBuf] = w;
+    u = wide_product<SF, WT, (I - 1) & 1>(at, u, r.own, gpt, true) *
+        read_lane(r.inv_lane, I);
+    acc += u;
+#pragma unroll
+    for (int t = 0; t < MA - I; ++t)
+        v[t] = fma(frechet_coef(I, t), u, v[t]);
+    if constexpr (I < MA)
+        u_chain<SF, WT, MA, I + 1>(u, acc, v, at, r, gpt);
+}
+
+// Exchange T (0..MA-1) of the forward-term chain on buffer (MA + T) & 1:
+// B += f_theta v_T w[c] over the WE extended slots, then
+// w <- (A w) / (T + 1) over the WA matrix slots (not after the last).
+template <int SE, int WE, int WA, int MA, int T>
+__device__ __forceinline__ void w_chain(double &w, const double (&v)[MA],
+                                        double (&B)[SE],
+                                        const double (&a)[WA],
+                                        const RevArgs &r,
+                                        const double *const (&gpe)[SE])
+{
+    constexpr int P = (MA + T) & 1;
+    r.own[P * kBuf] = w;
     __syncthreads();
-    double g[SE];
+    double g[WE];
     g[0] = w;
     g[1] = swap_pair(w);
 #pragma unroll
-    for (int sl = 2; sl < SE; ++sl)
+    for (int sl = 2; sl < WE; ++sl)
         g[sl] = gpe[sl][P * kBuf];
+    const double cv = r.fth * v[T];
 #pragma unroll
-    for (int sl = 0; sl < SE; ++sl)
+    for (int sl = 0; sl < WE; ++sl)
         B[sl] = fma(cv, g[sl], B[sl]);
-    if (!PRODUCT)
-        return 0.0;
-    double y0 = a[0] * g[0];
-    double y1 = a[1] * g[1];
+    if constexpr (T + 1 < MA) {
+        double y0 = a[0] * g[0];
+        double y1 = a[1] * g[1];
 #pragma unroll
-    for (int sl = 2; sl < SF; ++sl) {
-        if (sl & 1)
-            y1 = fma(a[sl], g[sl], y1);
-        else
-            y0 = fma(a[sl], g[sl], y0);
+        for (int sl = 2; sl < WA; ++sl) {
+            if (sl & 1)
+                y1 = fma(a[sl], g[sl], y1);
+            else
+                y0 = fma(a[sl], g[sl], y0);
+        }
+        w = (y0 + y1) * read_lane(r.inv_lane, T + 1);
+        w_chain<SE, WE, WA, MA, T + 1>(w, v, B, a, r, gpe);
     }
-    return y0 + y1;
 }
 
-// Reverse of one exponential with input y; enters and leaves with buffer
-// 0 next (2 MA exchanges).  lam: output cotangent in, input cotangent out.
-template <int SF, int SE, int MA>
-__device__ __forceinline__ void reverse_exp(
-    double y, double &lam, double (&B)[SE], const ExpArgs &e, int row,
-    double factor, double fth, double inv_lane, double *own,
-    const double *const (&gpe)[SE], const double *const (&gpt)[SF],
-    const int (&src)[SF])
+// Transposed half of one reverse exponential for a wavefront whose rows
+// use at most WT transposed slots: lambda <- exp(A)^T lambda and the
+// row-local v_t.  Starts on buffer 0, ends before buffer MA & 1.
+template <int SF, int WT, int MA>
+__device__ __forceinline__ void reverse_u(double &lam, double (&v)[MA],
+                                          const RevArgs &r,
+                                          const double *const (&gpt)[SF],
+                                          const uint32_t (&src)[SF / 2])
 {
-    static_assert(MA % 2 == 0, "exchanges are paired");
-    // row of A^T from the staged values of A
-    double at[SF];
+    double at[WT];
 #pragma unroll
-    for (int sl = 0; sl < SF; ++sl)
+    for (int sl = 0; sl < WT; ++sl)
         at[sl] = 0.0;
-    for (int m = 0; m < e.M; ++m) {
-        const double f = read_lane(factor, e.base + m);
-        const double *plane = e.lv + (size_t)m * SF * e.NP;
-        double vv[SF];
+    for (int m = 0; m < r.M; ++m) {
+        const double f = read_lane(r.factor, r.base + m);
+        const double *plane = r.lv + (size_t)m * SF * r.NP;
+        double vv[WT];
 #pragma unroll
-        for (int sl = 0; sl < SF; ++sl)
-            vv[sl] = plane[src[sl] >= 0 ? src[sl] : 0];
+        for (int sl = 0; sl < WT; ++sl)
+            vv[sl] = plane[(src[sl >> 1] >> (16 * (sl & 1))) & 0xFFFF];
 #pragma unroll
-        for (int sl = 0; sl < SF; ++sl)
-            at[sl] = fma(f, src[sl] >= 0 ? vv[sl] : 0.0, at[sl]);
+        for (int sl = 0; sl < WT; ++sl)
+            at[sl] = fma(f, vv[sl], at[sl]);
     }
-    // u chain (transposed products), lambda and v_t
     double u = lam;
     double acc = lam;
-    double v[MA];
 #pragma unroll
     for (int t = 0; t < MA; ++t)
         v[t] = frechet_coef(0, t) * u;
-#pragma unroll
-    for (int i = 1; i <= MA; i += 2) {
-        u = wide_product<SF, SF, 0>(at, u, own, gpt, true) *
-            read_lane(inv_lane, i);
-        acc += u;
-#pragma unroll
-        for (int t = 0; t < MA; ++t)
-            if (t < MA - i)
-                v[t] = fma(frechet_coef(i, t), u, v[t]);
-        u = wide_product<SF, SF, 1>(at, u, own, gpt, true) *
-            read_lane(inv_lane, i + 1);
-        acc += u;
-#pragma unroll
-        for (int t = 0; t < MA; ++t)
-            if (t < MA - i - 1)
-                v[t] = fma(frechet_coef(i + 1, t), u, v[t]);
-    }
+    u_chain<SF, WT, MA, 1>(u, acc, v, at, r, gpt);
     lam = acc;
-    // row of A
-    double a[SF];
+}
+
+// Forward-term half for a wavefront whose rows use at most WE extended
+// slots: B += f_theta sum_t v_t w_t^T on the extended pattern.
+template <int SF, int SE, int WE, int MA>
+__device__ __forceinline__ void reverse_w(double y, const double (&v)[MA],
+                                          double (&B)[SE],
+                                          const RevArgs &r,
+                                          const double *const (&gpe)[SE])
+{
+    constexpr int WA = WE < SF ? WE : SF;
+    double a[WA];
 #pragma unroll
-    for (int sl = 0; sl < SF; ++sl)
+    for (int sl = 0; sl < WA; ++sl)
         a[sl] = 0.0;
-    for (int m = 0; m < e.M; ++m) {
-        const double f = read_lane(factor, e.base + m);
+    for (int m = 0; m < r.M; ++m) {
+        const double f = read_lane(r.factor, r.base + m);
         const double2 *pv =
-            reinterpret_cast<const double2 *>(e.lv + (size_t)m * SF * e.NP) +
-            row;
-        double2 vv[SF / 2];
+            reinterpret_cast<const double2 *>(r.lv +
+                                              (size_t)m * SF * r.NP) +
+            r.row;
+        double2 vv[WA / 2];
 #pragma unroll
-        for (int pr = 0; pr < SF / 2; ++pr)
-            vv[pr] = pv[pr * e.NP];
+        for (int pr = 0; pr < WA / 2; ++pr)
+            vv[pr] = pv[pr * r.NP];
 #pragma unroll
-        for (int pr = 0; pr < SF / 2; ++pr) {
+        for (int pr = 0; pr < WA / 2; ++pr) {
             a[2 * pr] = fma(f, vv[pr].x, a[2 * pr]);
             a[2 * pr + 1] = fma(f, vv[pr].y, a[2 * pr + 1]);
         }
     }
-    // w chain with the accumulation of f_theta Abar
     double w = y;
-#pragma unroll
-    for (int t = 0; t < MA; t += 2) {
-        w = exchange_accumulate<SF, SE, 0, true>(w, fth * v[t], B, a, own,
-                                                 gpe) *
-            read_lane(inv_lane, t + 1);
-        if (t + 2 < MA)
-            w = exchange_accumulate<SF, SE, 1, true>(w, fth * v[t + 1], B,
-                                                     a, own, gpe) *
-                read_lane(inv_lane, t + 2);
+    w_chain<SE, WE, WA, MA, 0>(w, v, B, a, r, gpe);
+}
+
+// reverse_u for the smallest even class WT >= wt (wave-uniform)
+template <int SF, int MA, int WT = 2>
+__device__ __forceinline__ void reverse_u_dispatch(
+    int wt, double &lam, double (&v)[MA], const RevArgs &r,
+    const double *const (&gpt)[SF], const uint32_t (&src)[SF / 2])
+{
+    if constexpr (WT >= SF) {
+        reverse_u<SF, SF, MA>(lam, v, r, gpt, src);
+    } else {
+        if (wt <= WT)
+            reverse_u<SF, WT, MA>(lam, v, r, gpt, src);
         else
-            exchange_accumulate<SF, SE, 1, false>(w, fth * v[t + 1], B, a,
-                                                  own, gpe);
+            reverse_u_dispatch<SF, MA, WT + 2>(wt, lam, v, r, gpt, src);
+    }
+}
+
+// reverse_w for the smallest even class WE >= we (wave-uniform)
+template <int SF, int SE, int MA, int WE = 2>
+__device__ __forceinline__ void reverse_w_dispatch(
+    int we, double y, const double (&v)[MA], double (&B)[SE],
+    const RevArgs &r, const double *const (&gpe)[SE])
+{
+    if constexpr (WE >= SE) {
+        reverse_w<SF, SE, SE, MA>(y, v, B, r, gpe);
+    } else {
+        if (we <= WE)
+            reverse_w<SF, SE, WE, MA>(y, v, B, r, gpe);
+        else
+            reverse_w_dispatch<SF, SE, MA, WE + 2>(we, y, v, B, r, gpe);
     }
 }
 
@@ -285,17 +332,24 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
 #pragma unroll
     for (int sl = 0; sl < SF; ++sl)
         gpf[sl] = gpe[sl];
-    // recomputation slot class: the wavefront's widest row of the matrices,
-    // capped at the staged slots (columns beyond carry derivative entries
-    // only)
+    // slot classes of this wavefront: the matrices' widest row (capped at
+    // the staged slots; columns beyond carry derivative entries only) for
+    // the recomputation, the extended pattern for the forward-term chain
     int ws_f = wave_max(cnt_f > 0 ? cnt_f : cnt_e);
     ws_f = ws_f < staged ? ws_f : staged;
+    const int ws_e = wave_max(cnt_e < slots ? cnt_e : slots);
 
     // transposed pattern: gathers, and where each A^T value lives among the
     // staged values (row c, slot s' of A with column r)
     const int cnt_t = p.col_t[row];
+    const int ws_t = wave_max(cnt_t < d.slots_t ? cnt_t : d.slots_t);
+    // packed offsets (two per register) of the A^T values in an m-plane of
+    // the staged values; unused and missing entries name p.zero_entry
     const double *gpt[SF];
-    int src[SF];
+    uint32_t src[SF / 2];
+#pragma unroll
+    for (int sl = 0; sl < SF / 2; ++sl)
+        src[sl] = 0;
 #pragma unroll
     for (int sl = 0; sl < SF; ++sl) {
         int c = row;
@@ -316,7 +370,10 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
                     }
         }
         gpt[sl] = xb0 + (c & (kBuf - 1));
-        src[sl] = s < 0 ? -1 : (s >> 1) * 2 * NP + c * 2 + (s & 1);
+        const uint32_t off = s < 0 ? (uint32_t)p.zero_entry
+                                   : (uint32_t)((s >> 1) * 2 * NP + c * 2 +
+                                                (s & 1));
+        src[sl >> 1] |= off << (16 * (sl & 1));
     }
 
     double srow = 0.0;
@@ -373,7 +430,8 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
             }
         }
         if (parity) {
-            // the reverse sweep starts on buffer 0: finish every read of it
+            // each reverse exponential starts on buffer 0 (and, with 2 MA
+            // exchanges, ends ready for buffer 0): finish every read of it
             __syncthreads();
             parity = 0;
         }
@@ -385,11 +443,19 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
             for (int j = 0; j < J; ++j) {
                 const double y =
                     ws_load(ws + ((size_t)(k - k0) * J + j) * NP + row);
-                e.base = j * M;
-                const double fth =
-                    tg >= 0 ? read_lane(factor, j * M + tg) : 0.0;
-                reverse_exp<SF, SE, MA>(y, lam, B, e, row, factor, fth,
-                                        inv_lane, own, gpe, gpt, src);
+                RevArgs r;
+                r.lv = lv;
+                r.NP = NP;
+                r.M = M;
+                r.base = j * M;
+                r.row = row;
+                r.factor = factor;
+                r.fth = tg >= 0 ? read_lane(factor, j * M + tg) : 0.0;
+                r.inv_lane = inv_lane;
+                r.own = own;
+                double v[MA];
+                reverse_u_dispatch<SF, MA>(ws_t, lam, v, r, gpt, src);
+                reverse_w_dispatch<SF, SE, MA>(ws_e, y, v, B, r, gpe);
             }
             lam = fma(k == 0 ? 0.5 * d.h : d.h, srow, lam);
         }
@@ -421,11 +487,28 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
     }
 }
 
+// Offset (within one staged plane) of a value that is 0 for every matrix:
+// slot SF - 1 when the matrices use fewer slots, else slot 0 of a padding
+// row; -1 if there is none.
+int zero_entry(int SF, int slots_f, int n, int NP)
+{
+    if (slots_f < SF)
+        return ((SF - 1) >> 1) * 2 * NP + ((SF - 1) & 1);
+    if (NP > n)
+        return (NP - 1) * 2;
+    return -1;
+}
+
 template <int SF, int SE, int MA>
-int launch(const AWParams &p, hipStream_t stream)
+int launch(AWParams p, hipStream_t stream)
 {
     const int W = (p.d.n + kWave - 1) / kWave;
     const int NP = W * kWave;
+    p.zero_entry = zero_entry(SF, p.slots_f, p.d.n, NP);
+    if (p.zero_entry < 0 || p.zero_entry >= 65536)
+        return set_error(SSMX_EINVAL,
+                         "n=%d: no zero staged entry for %d matrix slots",
+                         p.d.n, p.slots_f);
     const size_t lds =
         ((size_t)p.d.n_gen * SF * NP + 2 * kBuf + W) * sizeof(double);
     if (lds > 160 * 1024)
@@ -453,8 +536,11 @@ int launch(const AWParams &p, hipStream_t stream)
 template <int SF, int SE>
 int pick_degree(const AWParams &p, hipStream_t s)
 {
+    // the planned degree, at least 8 (a higher degree only adds accuracy)
     if (p.d.taylor_adj <= 8)
         return launch<SF, SE, 8>(p, s);
+    if (p.d.taylor_adj == 9)
+        return launch<SF, SE, 9>(p, s);
     return launch<SF, SE, 10>(p, s);
 }
 
@@ -492,16 +578,23 @@ int ssmx_adjoint_wide(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
     p.lambda_out = lambda_out;
     const int sf = p.slots_f > desc->slots_t ? p.slots_f : desc->slots_t;
     const int se = desc->slots;
-    if (sf <= 8 && se <= 16)
+    const int NP = (desc->n + kWave - 1) / kWave * kWave;
+    // staged slots SF (matrix and transposed) and extended slots SE; a
+    // variant needs a zero staged entry (zero_entry)
+    auto fits = [&](int SF) {
+        return sf <= SF && zero_entry(SF, p.slots_f, desc->n, NP) >= 0;
+    };
+    if (fits(8) && se <= 16)
         return pick_degree<8, 16>(p, stream);
-    if (sf <= 12 && se <= 20)
+    if (fits(12) && se <= 20)
         return pick_degree<12, 20>(p, stream);
-    if (sf <= 16 && se <= 24)
-        return pick_degree<16, 24>(p, stream);
-    if (sf <= 16 && se <= 32)
+    if (fits(16) && se <= 22)
+        return pick_degree<16, 22>(p, stream);
+    if (fits(16) && se <= 32)
         return pick_degree<16, 32>(p, stream);
     return set_error(SSMX_EINVAL,
                      "n > 64 adjoint supports at most 16 matrix / transposed "
-                     "slots and 32 extended slots (got %d / %d / %d)",
+                     "slots (15 when n is a multiple of 64) and 32 extended "
+                     "slots (got %d / %d / %d)",
                      p.slots_f, desc->slots_t, desc->slots);
 }
