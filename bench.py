@@ -80,6 +80,8 @@ def parse():
     p.add_argument("--protocol-count", type=int, default=16384)
     p.add_argument("--protocol-steps", type=int, default=1,
                    help="timed graph replays of the protocol batch")
+    p.add_argument("--no-secondary", action="store_true",
+                   help="skip the n = 19 and 3-level sweep lines")
     p.add_argument("--no-hyperfine", action="store_true",
                    help="skip the 21-level hyperfine (n = 441) leg")
     p.add_argument("--hyperfine-only", action="store_true")
@@ -368,6 +370,54 @@ def hyperfine_leg(args):
     return legs
 
 
+def secondary_leg(name):
+    """Steps/s of another sweep config (one full launch, inputs resident)."""
+    import numpy as np
+    import torch
+    from superspinsim_amd.propagator import device_tensor
+    from superspinsim_amd.sweep import pack_waveforms
+    wl = workload(name, None)
+    sim = wl.simulator()
+    waves = wl.waveforms()
+    n_inst = len(waves)
+    sim.run(waves, wl.rho0, 0.0, wl.time_step * (1 + 1e-12), wl.time_step,
+            obs=wl.obs, read="obs")
+    plan = dict(sim.last_plan)
+    prop, program, m, sub = sim.plan(waves, wl.time_step)
+    d_waves, d_segs = pack_waveforms(sim.kernel_waves(waves, program),
+                                     len(program))
+    d_rho = device_tensor(np.repeat(wl.rho0[None], n_inst, axis=0))
+    out = torch.empty((n_inst, wl.n_out, 1), dtype=torch.float64,
+                      device="cuda")
+
+    def launch():
+        prop.run(n_inst=n_inst, fine=wl.fine, n_out=wl.n_out,
+                 h=wl.time_step / wl.fine, t0=0.0, dt=wl.time_step,
+                 rho0=d_rho, waves=d_waves, segments=d_segs, taylor=m,
+                 substeps=sub, read="obs", obs=wl.obs, out=out)
+
+    launch()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    launch()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    steps = n_inst * wl.steps_per_instance()
+    plan["steps"] = wl.steps_per_instance()
+    flops = sim.step_flops(plan) * steps / (ev[0].elapsed_time(ev[1]) / 1e3)
+    return {"workload": wl.name, "instances": n_inst,
+            "superoperator_n": wl.n,
+            "steps_per_instance": wl.steps_per_instance(),
+            "taylor_degree": m, "ell_slots": plan["slots"],
+            "s_per_sweep": wall, "steps_per_s": steps / wall,
+            "achieved_tflops": flops / 1e12,
+            "finite": bool(np.all(np.isfinite(out.cpu().numpy())))}
+
+
 def hyperfine_adjoint_leg(args):
     """Forward + adjoint of the 21-level sweep (n = 441): gradients of the
     integrated fluorescence over the 7-level and 14N parameters."""
@@ -568,6 +618,10 @@ def main():
         result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]
     if not args.no_protocols and args.config == "odmr7":
         result["protocols"] = protocol_leg(args)
+    if args.config == "odmr7" and not args.no_secondary:
+        # the compressed 7-level vectorisation and the 3-level sweep
+        result["secondary"] = [secondary_leg("odmr7-n19"),
+                               secondary_leg("odmr3")]
     if not args.no_hyperfine and args.config == "odmr7":
         result["hyperfine"] = hyperfine_leg(args)
     if not args.no_hyperfine_adjoint and args.config == "odmr7":

@@ -163,3 +163,58 @@ def test_wide_dropin_generate_simulator_vs_oracle(gpu):
     err = np.max(np.abs(d_dev - d_ref)) / np.max(np.abs(d_ref))
     print(f"  rel err {err:.3e}")
     assert err <= TOL
+
+
+def test_wide_rotating_frame_vs_oracle(gpu):
+    """Rotating frame at n = 98 (node-wise generators, eigen-step) against
+    the oracle's rotating path on the same tables.
+
+    L0 is given by its real block eigen-data (45 complex pairs, 8 real
+    eigenvalues) in the standard basis (V = I), so the interaction-picture
+    generators keep the block pattern of two random sparse generators; one
+    has a constant coefficient, the other a microwave tone."""
+    from superspinsim_amd.sweep import BatchSimulator, Waveform
+    from oracle.reference_simulate import flatten
+    rng = np.random.default_rng(21)
+    nd, ns = 45, 8
+    n = 2 * nd + ns                                  # 98 = 2 * 7 * 7
+    doubles = np.stack([-rng.uniform(1e6, 1e8, nd),
+                        rng.uniform(-2e10, 2e10, nd)], axis=1)
+    singles = -rng.uniform(1e5, 1e7, ns)
+    units = [(2 * k, 2 * k + 2) for k in range(nd)] + \
+        [(2 * nd + k, 2 * nd + k + 1) for k in range(ns)]
+    gens = np.zeros((2, n, n))
+    for g in range(2):
+        for (r0, r1) in units:
+            for c in rng.choice(len(units), 3, replace=False):
+                c0, c1 = units[c]
+                gens[g, r0:r1, c0:c1] = rng.standard_normal((r1 - r0,
+                                                             c1 - c0))
+    gens *= 2e8 / np.abs(gens).sum(axis=2).max()
+    eye = np.eye(n)
+    F, dt, n_out = 4, 8e-11, 3                       # h = 20 ps
+    freqs = [2.1e9, 2.9e9]
+    waves = []
+    for f in freqs:
+        w = Waveform(2, base=[1.0, 0.0])
+        w.tone(1, 0.7, f)
+        waves.append(w)
+    rho = rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7))
+    x0 = flatten(rho, None, False, n)
+    sim = BatchSimulator(gens, 2, F, use_rotating=True, vectors_real=eye,
+                         inv_vectors_real=eye, doubles=doubles,
+                         singles=singles)
+    t_end = n_out * dt * (1 + 1e-12)
+    out = sim.run(waves, x0, 0.0, t_end, dt, read="full").cpu().numpy()
+    assert sim.last_plan["n"] == n > 64
+    for k, w in enumerate(waves):
+        osim = oracle_gen(w.sampler(), gens, None, use_hermitian=False,
+                          use_rotating=True, vectors_real=eye,
+                          inv_vectors_real=eye, doubles=doubles,
+                          singles=singles, number_of_exponentials=2,
+                          number_of_fine_divisions=F, backend="numpy")
+        _, dens = osim(rho, 0.0, t_end, dt)
+        ref = np.stack([flatten(d, None, False, n) for d in dens])
+        err = np.max(np.abs(out[k] - ref)) / np.max(np.abs(ref))
+        print(f"  f={freqs[k]:.2e}: rotating n=98 rel err {err:.3e}")
+        assert err <= TOL
