@@ -30,14 +30,23 @@ inline int set_error(int code, const char *fmt, ...)
     return code;
 }
 
-// Value of `v` held by lane ^ 1 (DPP quad_perm [1, 0, 3, 2]).
-__device__ __forceinline__ double swap_pair(double v)
+// Value of `v` from another lane of the same quad by a DPP quad
+// permutation: CTRL packs the source lane of quad positions 0..3 in bit
+// pairs (quad_perm [s0, s1, s2, s3] = s0 | s1 << 2 | s2 << 4 | s3 << 6).
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double v)
 {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0xB1, 0xF,
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF,
                                             0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0xB1, 0xF,
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF,
                                             0xF, false);
     return __hiloint2double(hi, lo);
+}
+
+// Value of `v` held by lane ^ 1.
+__device__ __forceinline__ double swap_pair(double v)
+{
+    return quad_perm<0xB1>(v);  // [1, 0, 3, 2]
 }
 
 // Broadcast lane `src` (wave-uniform) of `v` to the whole wavefront.
@@ -74,6 +83,83 @@ __device__ __forceinline__ void wave_exchange()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The quad layout: slot 1 of row r is column r ^ 1, slots 2 and 3 are the
+// even and the odd row of the other pair of r's quad, (r & ~1) ^ 2 and
+// (r & ~1) ^ 3.  True (wave-uniform) when every lane's columns follow it;
+// slot 1 is then read by DPP and slots 2, 3 by one 16-byte LDS read.
+template <int SLOTS>
+__device__ __forceinline__ bool quad_layout(const int (&col)[SLOTS],
+                                            int lane, bool partner)
+{
+    if constexpr (SLOTS < 4) {
+        return false;
+    } else {
+        const int other = (lane & ~1) ^ 2;
+        const bool mine = col[1] == (lane ^ 1) && col[2] == other &&
+                          col[3] == (other | 1);
+        return partner &&
+               __builtin_amdgcn_readfirstlane(__all(mine ? 1 : 0)) != 0;
+    }
+}
+
+// y = A v with lane r holding ELL row r of A (values a[], columns col[]) and
+// component r of v: one Taylor product.  Slot 0 is the lane's own
+// component.  With QUAD (the quad layout), slot 1 comes from lane r ^ 1 by
+// DPP and slots 2, 3 (the other pair of the quad) by one 16-byte LDS read;
+// otherwise slot 1 is lane ^ 1 by DPP when `partner` is set.  The other
+// slots are gathered from `xbuf` (one double per lane of the wavefront,
+// 16-byte aligned) after every lane has written its component;
+// LDS instructions of a wavefront execute in order, so a wavefront-scope
+// fence is the only barrier.  Even and odd slots are summed in two chains in
+// slot order, so the rounding depends on the row's values and columns only,
+// not on how the components were fetched.
+template <int SLOTS, bool QUAD>
+__device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
+                                            const int (&col)[SLOTS],
+                                            double v, double *xbuf, int lane,
+                                            bool partner)
+{
+    double g[SLOTS];
+    g[0] = v;
+    if constexpr (QUAD) {
+        // One LDS instruction for the two slots instead of two gathers.
+        // Reading them by DPP (two moves per double) instead was measured
+        // no faster than the gathers: the moves cost as much issue as the
+        // LDS instructions they remove.
+        static_assert(SLOTS >= 4, "a quad row has at least 4 slots");
+        g[1] = swap_pair(v);
+        xbuf[lane] = v;
+        wave_exchange();
+        const double2 o =
+            reinterpret_cast<const double2 *>(xbuf)[(lane >> 1) ^ 1];
+        g[2] = o.x;
+        g[3] = o.y;
+#pragma unroll
+        for (int sl = 4; sl < SLOTS; ++sl)
+            g[sl] = xbuf[col[sl]];
+        wave_exchange();
+    } else {
+        xbuf[lane] = v;
+        wave_exchange();
+        if (SLOTS > 1)
+            g[1] = partner ? swap_pair(v) : xbuf[col[1]];
+#pragma unroll
+        for (int sl = 2; sl < SLOTS; ++sl)
+            g[sl] = xbuf[col[sl]];
+        wave_exchange();
+    }
+    double y0 = a[0] * g[0];
+    double y1 = 0.0;
+#pragma unroll
+    for (int sl = 1; sl < SLOTS; ++sl) {
+        if (sl & 1)
+            y1 = fma(a[sl], g[sl], y1);
+        else
+            y0 = fma(a[sl], g[sl], y0);
+    }
+    return y0 + y1;
 }
 
 // Node time of global output interval i, fine division r, node c, in the

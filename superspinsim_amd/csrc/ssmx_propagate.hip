@@ -21,6 +21,10 @@
 //  - slot 0 of every ELL row is the diagonal: no exchange;
 //  - slot 1 is the partner column r ^ 1 (the Re/Im pair of a coherence, or
 //    a 2x2 eigen-block): exchanged with a DPP quad permutation (VALU);
+//  - in the quad basis order (slots 1..3 of every row are the other lanes
+//    of the row's quad) slots 2 and 3, the other pair of the quad, come
+//    from one 16-byte LDS read (ssmx::ell_apply); the kernel recognises
+//    that layout from the column table;
 //  - slots 2.. are arbitrary columns: each lane writes its component to a
 //    per-wavefront LDS buffer once, then reads its columns with ds_read_b64
 //    (LDS instructions of one wavefront execute in order, so no barrier is
@@ -36,13 +40,14 @@
 
 namespace {
 
+using ssmx::ell_apply;
 using ssmx::kWave;
 using ssmx::node_time;
+using ssmx::quad_layout;
 using ssmx::read_lane;
 using ssmx::set_error;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
-using ssmx::wave_exchange;
 using ssmx::wave_sum;
 
 constexpr int kWavesPerBlock = 8;
@@ -113,41 +118,20 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
     return d.h * cw;
 }
 
-template <int SLOTS>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
-    Params p)
+// One instance on one wavefront, after the kernel has staged the generator
+// values (`vals`, [m][slot / 2][lane][slot % 2]) and loaded the lane's ELL
+// columns.  QUAD: slots 1..3 are the other lanes of the quad (ell_apply).
+template <int SLOTS, bool QUAD>
+__device__ __forceinline__ void propagate_instance(const Params &p,
+                                                   const double *vals,
+                                                   double *xbuf, int M,
+                                                   int inst,
+                                                   const int (&col)[SLOTS])
 {
-    // LDS: generator values staged as [m][slot / 2][lane][slot % 2] (one
-    // ds_read_b128 returns two slots of a row), then one 64-double exchange
-    // buffer per wavefront.
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
-    extern __shared__ double lds[];
     const ssmx_propagate_desc &d = p.d;
     const int lane = threadIdx.x & (kWave - 1);
     const bool rotating = (d.flags & SSMX_ROTATING) != 0;
-    const int M = (rotating ? d.n_node : 1) * d.n_gen;  // matrices
-    {
-        const int nval = M * SLOTP * kWave;
-        for (int e = threadIdx.x; e < nval; e += blockDim.x) {
-            const int half = e & 1;
-            const int row = (e >> 1) % kWave;
-            const int pair = (e / (2 * kWave)) % (SLOTP / 2);
-            const int m = e / (kWave * SLOTP);
-            const int slot = 2 * pair + half;
-            lds[e] = slot < d.slots
-                         ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
-                                     row]
-                         : 0.0;
-        }
-        __syncthreads();
-    }
-
-    const int inst = __builtin_amdgcn_readfirstlane(
-        (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave));
-    if (inst >= d.n_inst)
-        return;  // whole wavefront; no barrier follows
-    double *xbuf = lds + M * SLOTP * kWave + (threadIdx.x / kWave) * kWave;
-
     const int n = d.n;
     const int J = d.n_exp;
     const bool wave_mode = (d.flags & SSMX_COEF_WAVE) != 0;
@@ -162,15 +146,6 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     else
         q.coef = p.coef + (size_t)inst * d.coef_inst_stride;
 
-    // Slot 0 is the diagonal by definition (ell_col[0][row] holds the row's
-    // used-slot count); slot 1 is the partner r ^ 1 when `partner` is set;
-    // other slots hold column indices.  Unused slots point at the row
-    // itself with value 0, so every slot is read without predication.
-    int col[SLOTS];
-    col[0] = lane;
-#pragma unroll
-    for (int s = 1; s < SLOTS; ++s)
-        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
     // Lane t holds 1/t: the Taylor recursion broadcasts it with readlane
     // (a scalar memory load or a division would sit in the dependent
     // chain of every term).
@@ -207,30 +182,6 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     const double inv_sub = 1.0 / (double)d.substeps;
     double xint = 0.0;
     const double half_h = 0.5 * d.h;
-
-    // y = A v for the current ELL row values a[] (one Taylor product).
-    auto product = [&](const double (&a)[SLOTS], double v) -> double {
-        double g[SLOTS];
-        g[0] = v;
-        xbuf[lane] = v;
-        wave_exchange();
-        if (SLOTS > 1)
-            g[1] = use_partner ? swap_pair(v) : xbuf[col[1]];
-#pragma unroll
-        for (int sl = 2; sl < SLOTS; ++sl)
-            g[sl] = xbuf[col[sl]];
-        wave_exchange();
-        double y0 = a[0] * g[0];
-        double y1 = 0.0;
-#pragma unroll
-        for (int sl = 1; sl < SLOTS; ++sl) {
-            if (sl & 1)
-                y1 = fma(a[sl], g[sl], y1);
-            else
-                y0 = fma(a[sl], g[sl], y0);
-        }
-        return y0 + y1;
-    };
 
     // Checkpoints: with ckpt_every = K > 0, state_out receives the state
     // before steps 0, K, 2K, ... instead of the final state.
@@ -271,7 +222,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                 for (int m = 0; m < M; ++m) {
                     const double f = read_lane(factor, base + m);
                     const double2 *v =
-                        reinterpret_cast<const double2 *>(lds +
+                        reinterpret_cast<const double2 *>(vals +
                                                           m * SLOTP * kWave) +
                         lane;
 #pragma unroll
@@ -294,7 +245,9 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
                         // no contraction: the adjoint's recomputation
                         // repeats these two roundings
 #pragma clang fp contract(off)
-                        term = product(a, term) * read_lane(inv_lane, t);
+                        term = ell_apply<SLOTS, QUAD>(a, col, term, xbuf,
+                                                      lane, use_partner) *
+                               read_lane(inv_lane, t);
                         acc += term;
                     }
                     x = acc;
@@ -334,6 +287,59 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     }
     if (p.state_out && ckpt_every == 0 && lane < n)
         p.state_out[(size_t)inst * n + lane] = x;
+}
+
+template <int SLOTS>
+__global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
+    Params p)
+{
+    // LDS: generator values staged as [m][slot / 2][lane][slot % 2] (one
+    // ds_read_b128 returns two slots of a row), then one 64-double exchange
+    // buffer per wavefront.
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    extern __shared__ double lds[];
+    const ssmx_propagate_desc &d = p.d;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
+    const int M = (rotating ? d.n_node : 1) * d.n_gen;  // matrices
+    {
+        const int nval = M * SLOTP * kWave;
+        for (int e = threadIdx.x; e < nval; e += blockDim.x) {
+            const int half = e & 1;
+            const int row = (e >> 1) % kWave;
+            const int pair = (e / (2 * kWave)) % (SLOTP / 2);
+            const int m = e / (kWave * SLOTP);
+            const int slot = 2 * pair + half;
+            lds[e] = slot < d.slots
+                         ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
+                                     row]
+                         : 0.0;
+        }
+        __syncthreads();
+    }
+
+    const int inst = __builtin_amdgcn_readfirstlane(
+        (int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave));
+    if (inst >= d.n_inst)
+        return;  // whole wavefront; no barrier follows
+    double *xbuf = lds + M * SLOTP * kWave + (threadIdx.x / kWave) * kWave;
+
+    // Slot 0 is the diagonal by definition (ell_col[0][row] holds the row's
+    // used-slot count); slot 1 is the partner r ^ 1 when `partner` is set;
+    // other slots hold column indices.  Unused slots point at the row
+    // itself with value 0, so every slot is read without predication.
+    int col[SLOTS];
+    col[0] = lane;
+#pragma unroll
+    for (int s = 1; s < SLOTS; ++s)
+        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
+    if constexpr (SLOTS >= 4) {
+        if (quad_layout(col, lane, d.partner != 0)) {
+            propagate_instance<SLOTS, true>(p, lds, xbuf, M, inst, col);
+            return;
+        }
+    }
+    propagate_instance<SLOTS, false>(p, lds, xbuf, M, inst, col);
 }
 
 template <int SLOTS>

@@ -18,13 +18,14 @@
 
 namespace {
 
+using ssmx::ell_apply;
 using ssmx::kWave;
 using ssmx::node_time;
+using ssmx::quad_layout;
 using ssmx::read_lane;
 using ssmx::set_error;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
-using ssmx::wave_exchange;
 using ssmx::wave_sum;
 
 constexpr int kMaxWavesPerBlock = 8;
@@ -83,41 +84,21 @@ __device__ double step_factor_q(const QParams &p, const ssmx_wave *wv,
     return d.h * cw;
 }
 
-template <int SLOTS>
-__global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
-    QParams p)
+// The queue loop of one wavefront, after the kernel has staged the
+// generator values (`vals`) and loaded the lane's ELL columns.  QUAD: slots
+// 1..3 are the other lanes of the quad (ell_apply).
+template <int SLOTS, bool QUAD>
+__device__ __forceinline__ void queue_loop(const QParams &p,
+                                           const double *vals, double *xbuf,
+                                           int M, const int (&col)[SLOTS])
 {
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
-    extern __shared__ double lds[];
     const ssmx_propagate_desc &d = p.d;
     const int lane = threadIdx.x & (kWave - 1);
     const bool rotating = (d.flags & SSMX_ROTATING) != 0;
-    const int M = (rotating ? d.n_node : 1) * d.n_gen;
-    {
-        const int nval = M * SLOTP * kWave;
-        for (int e = threadIdx.x; e < nval; e += blockDim.x) {
-            const int half = e & 1;
-            const int row = (e >> 1) % kWave;
-            const int pair = (e / (2 * kWave)) % (SLOTP / 2);
-            const int m = e / (kWave * SLOTP);
-            const int slot = 2 * pair + half;
-            lds[e] = slot < d.slots
-                         ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
-                                     row]
-                         : 0.0;
-        }
-        __syncthreads();
-    }
-    double *xbuf = lds + M * SLOTP * kWave + (threadIdx.x / kWave) * kWave;
-
     const int n = d.n;
     const int J = d.n_exp;
     const bool use_partner = d.partner != 0;
-    int col[SLOTS];
-    col[0] = lane;
-#pragma unroll
-    for (int s = 1; s < SLOTS; ++s)
-        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
     const double inv_lane = kInvQ[lane <= SSMX_MAX_TAYLOR ? lane : 0];
     double eig_own = 0.0, eig_part = 0.0;
     if (rotating) {
@@ -133,29 +114,6 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
     const int my_m = lane % M;
     const double inv_sub = 1.0 / (double)d.substeps;
     const double half_h = 0.5 * d.h;
-
-    auto product = [&](const double (&a)[SLOTS], double v) -> double {
-        double g[SLOTS];
-        g[0] = v;
-        xbuf[lane] = v;
-        wave_exchange();
-        if (SLOTS > 1)
-            g[1] = use_partner ? swap_pair(v) : xbuf[col[1]];
-#pragma unroll
-        for (int sl = 2; sl < SLOTS; ++sl)
-            g[sl] = xbuf[col[sl]];
-        wave_exchange();
-        double y0 = a[0] * g[0];
-        double y1 = 0.0;
-#pragma unroll
-        for (int sl = 1; sl < SLOTS; ++sl) {
-            if (sl & 1)
-                y1 = fma(a[sl], g[sl], y1);
-            else
-                y0 = fma(a[sl], g[sl], y0);
-        }
-        return y0 + y1;
-    };
 
     // Every pass takes one queue entry; the counter only grows, so every
     // wavefront leaves the loop once the n_inst entries are handed out.
@@ -194,7 +152,7 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
                 for (int m = 0; m < M; ++m) {
                     const double f = read_lane(factor, base + m);
                     const double2 *v =
-                        reinterpret_cast<const double2 *>(lds +
+                        reinterpret_cast<const double2 *>(vals +
                                                           m * SLOTP * kWave) +
                         lane;
 #pragma unroll
@@ -215,7 +173,9 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
                     double acc = x;
                     for (int t = 1; t <= d.taylor; ++t) {
 #pragma clang fp contract(off)
-                        term = product(a, term) * read_lane(inv_lane, t);
+                        term = ell_apply<SLOTS, QUAD>(a, col, term, xbuf,
+                                                      lane, use_partner) *
+                               read_lane(inv_lane, t);
                         acc += term;
                     }
                     x = acc;
@@ -237,6 +197,48 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
                 p.out[(size_t)inst * d.n_obs + 1] = v1;
         }
     }
+}
+
+template <int SLOTS>
+__global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
+    QParams p)
+{
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    extern __shared__ double lds[];
+    const ssmx_propagate_desc &d = p.d;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
+    const int M = (rotating ? d.n_node : 1) * d.n_gen;
+    {
+        const int nval = M * SLOTP * kWave;
+        for (int e = threadIdx.x; e < nval; e += blockDim.x) {
+            const int half = e & 1;
+            const int row = (e >> 1) % kWave;
+            const int pair = (e / (2 * kWave)) % (SLOTP / 2);
+            const int m = e / (kWave * SLOTP);
+            const int slot = 2 * pair + half;
+            lds[e] = slot < d.slots
+                         ? p.ell_val[((size_t)m * d.slots + slot) * kWave +
+                                     row]
+                         : 0.0;
+        }
+        __syncthreads();
+    }
+    double *xbuf = lds + M * SLOTP * kWave + (threadIdx.x / kWave) * kWave;
+
+    const bool use_partner = d.partner != 0;
+    int col[SLOTS];
+    col[0] = lane;
+#pragma unroll
+    for (int s = 1; s < SLOTS; ++s)
+        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
+    if constexpr (SLOTS >= 4) {
+        if (quad_layout(col, lane, use_partner)) {
+            queue_loop<SLOTS, true>(p, lds, xbuf, M, col);
+            return;
+        }
+    }
+    queue_loop<SLOTS, false>(p, lds, xbuf, M, col);
 }
 
 template <int SLOTS>

@@ -96,17 +96,152 @@ def width_order(pattern, partner=True):
     return order.astype(np.int64)
 
 
+def quad_columns(r):
+    """Columns of slots 1, 2, 3 of row r in the quad layout: r ^ 1, then
+    the even and the odd row of the other pair of r's quad."""
+    other = (r & ~1) ^ 2
+    return (r ^ 1, other, other | 1)
+
+
+def _quad_needs(pattern):
+    need = np.asarray(pattern, dtype=bool).copy()
+    np.fill_diagonal(need, False)
+    return need
+
+
+def quad_gathers(pattern):
+    """Largest number of columns of a row outside its quad (rows 4q..4q+3).
+
+    These are the single LDS gathers per product left in the quad layout,
+    beside the write and the 16-byte read of the quad's other pair.
+    """
+    need = _quad_needs(pattern)
+    n = need.shape[0]
+    grp = np.arange(n) // 4
+    outside = need & (grp[:, None] != grp[None, :])
+    return int(outside.sum(axis=1).max()) if n else 0
+
+
+def quad_order(pattern, pair_units=False, restarts=8, seed=0):
+    """Basis order that puts coupled rows into common quads (4q .. 4q + 3).
+
+    Within a quad every row reads its partner lane by DPP and the other
+    pair of the quad by one 16-byte LDS read, so each column outside its
+    quad costs one more LDS instruction.  The order minimises, lexicographically, the largest count of such
+    columns over the rows (the wavefront executes that many gathers per
+    product), the number of rows at that count, and their sum, by steepest
+    descent over swaps between quads, from the current order and from a few
+    seeded random starts.  ``pair_units`` keeps the rows (2k, 2k + 1) of the
+    current basis together as one unit (the rotating frame's 2x2
+    eigen-blocks), so pairs stay aligned.  With n not a multiple of 4 the
+    last quad is the partial one (an odd last row stays last).  Returns
+    ``order`` with ``new[k] = old[order[k]]``.
+    """
+    need = _quad_needs(pattern).astype(np.int64)
+    n = need.shape[0]
+    if n <= 4:
+        return np.arange(n, dtype=np.int64)
+    size = 2 if pair_units else 1
+    per_quad = 4 // size
+    U = (n + size - 1) // size
+    Q = (U + per_quad - 1) // per_quad
+    # unit u holds rows size*u .. ; a padding row n and a padding unit U
+    # (no couplings) fill partial quads
+    unit_rows = np.full((U + 1, size), n, dtype=np.int64)
+    for u in range(U):
+        rows = np.arange(size * u, min(size * u + size, n))
+        unit_rows[u, :rows.size] = rows
+    need_p = np.zeros((n + 1, n + 1), dtype=np.int64)
+    need_p[:n, :n] = need
+    Nu = need_p[:, unit_rows].sum(axis=2)            # (n + 1, U + 1)
+    base = need_p.sum(axis=1)
+    fixed_tail = 1 if (pair_units and n % 2) else 0
+    movable = U - fixed_tail
+    rng = np.random.default_rng(seed)
+    ii, jj = np.triu_indices(movable, 1)
+
+    def evaluate(slot_unit):
+        quads = slot_unit.reshape(Q, per_quad)
+        S = Nu[:, quads].sum(axis=2)                  # (n + 1, Q)
+        q_of_unit = np.zeros(U + 1, dtype=np.int64)
+        q_of_unit[quads.ravel()] = np.repeat(np.arange(Q), per_quad)
+        unit_of_row = np.full(n + 1, U, dtype=np.int64)
+        for u in range(U):
+            unit_of_row[unit_rows[u][unit_rows[u] < n]] = u
+        q_of_row = q_of_unit[unit_of_row]
+        rem = base - S[np.arange(n + 1), q_of_row]
+        rem[n] = 0
+        return quads, S, q_of_row, unit_of_row, rem
+
+    def key_of(rem):
+        m = rem.max()
+        return (int(m), int((rem == m).sum()), int(rem.sum()))
+
+    best = None
+    for start in range(restarts + 1):
+        order = np.arange(movable) if start == 0 else \
+            rng.permutation(movable)
+        slot_unit = np.full(Q * per_quad, U, dtype=np.int64)
+        slot_unit[:movable] = order
+        if fixed_tail:
+            slot_unit[movable] = U - 1
+        while True:
+            quads, S, q_of_row, unit_of_row, rem = evaluate(slot_unit)
+            cur = key_of(rem[:n])
+            qa, qb = ii // per_quad, jj // per_quad
+            ok = qa != qb
+            pi, pj, qa, qb = ii[ok], jj[ok], qa[ok], qb[ok]
+            ua, ub = slot_unit[pi], slot_unit[pj]
+            if pi.size == 0:
+                break
+            rowsA = unit_rows[quads[qa]].reshape(pi.size, -1)   # (P, 4)
+            rowsB = unit_rows[quads[qb]].reshape(pi.size, -1)
+            uA = unit_of_row[rowsA]
+            uB = unit_of_row[rowsB]
+            Sa_A = S[rowsA, qa[:, None]]
+            Sa_B = S[rowsA, qb[:, None]]
+            Sb_A = S[rowsB, qa[:, None]]
+            Sb_B = S[rowsB, qb[:, None]]
+            na_a, na_b = Nu[rowsA, ua[:, None]], Nu[rowsA, ub[:, None]]
+            nb_a, nb_b = Nu[rowsB, ua[:, None]], Nu[rowsB, ub[:, None]]
+            covA = np.where(uA == ua[:, None], Sa_B - na_b + na_a,
+                            Sa_A - na_a + na_b)
+            covB = np.where(uB == ub[:, None], Sb_A - nb_a + nb_b,
+                            Sb_B - nb_b + nb_a)
+            R = np.broadcast_to(rem, (pi.size, n + 1)).copy()
+            rows = np.arange(pi.size)[:, None]
+            R[rows, rowsA] = base[rowsA] - covA
+            R[rows, rowsB] = base[rowsB] - covB
+            R[:, n] = 0
+            mx = R.max(axis=1)
+            cnt = (R == mx[:, None]).sum(axis=1)
+            tot = R.sum(axis=1)
+            k = int(np.lexsort((tot, cnt, mx))[0])
+            if (int(mx[k]), int(cnt[k]), int(tot[k])) >= cur:
+                break
+            slot_unit[pi[k]], slot_unit[pj[k]] = \
+                slot_unit[pj[k]], slot_unit[pi[k]]
+        if best is None or cur < best[0]:
+            best = (cur, slot_unit.copy())
+    order = [int(r) for u in best[1] if u < U for r in unit_rows[u]
+             if r < n]
+    return np.array(order, dtype=np.int64)
+
+
 class EllStack:
     """Generators packed in one ELL pattern (the union of their non-zeros).
 
     ``mats`` has shape (M, n, n).  Slot 0 of row r is always column r (the
-    diagonal); with ``partner`` slot 1 is always column r ^ 1; the other
-    non-zeros of the union fill slots 2.. in column order.  ``col`` is int32
+    diagonal); with ``partner`` slot 1 is always column r ^ 1; with ``quad``
+    (n <= 64) slots 1, 2, 3 are always the other rows of r's quad,
+    ``quad_columns(r)``, for every row of the padded table (the kernels
+    recognise this layout and read those slots without LDS gathers).  The other non-zeros
+    of the union fill the next slots in column order.  ``col`` is int32
     (slots, NP), ``val`` float64 (M, slots, NP) with NP = 64 ceil(n / 64);
     unused slots point at column r with value 0, rows >= n are all zero.
     """
 
-    def __init__(self, mats, partner=True):
+    def __init__(self, mats, partner=True, quad=False):
         mats = np.asarray(mats, dtype=_f64)
         if mats.ndim != 3 or mats.shape[1] != mats.shape[2]:
             raise ValueError(f"generator stack must be (M, n, n), got "
@@ -116,19 +251,29 @@ class EllStack:
             raise ValueError(f"superoperator dimension {n} > "
                              f"{_lib.MAX_N_WIDE}")
         NP = padded_rows(n)
+        if quad and (not partner or NP != 64):
+            raise ValueError("the quad layout needs partner=True and n <= 64")
         union = np.any(mats != 0, axis=0)
-        fixed = 2 if partner else 1
+        fixed = 4 if quad else (2 if partner else 1)
         rows = []
         for r in range(n):
-            special = {r} | ({r ^ 1} if partner else set())
+            special = {r, *quad_columns(r)} if quad else \
+                ({r} | ({r ^ 1} if partner else set()))
             rows.append([c for c in np.nonzero(union[r])[0]
                          if c not in special])
         slots = fixed + max(len(c) for c in rows)
         col = np.tile(np.arange(NP, dtype=np.int32), (slots, 1))
         val = np.zeros((M, slots, NP), dtype=_f64)
+        if quad:
+            lanes = np.arange(NP, dtype=np.int32)
+            col[1:4, :] = quad_columns(lanes)
         for r in range(n):
             val[:, 0, r] = mats[:, r, r]
-            if partner:
+            if quad:
+                for k, c in enumerate(quad_columns(r), start=1):
+                    if c < n:
+                        val[:, k, r] = mats[:, r, c]
+            elif partner:
                 col[1, r] = r ^ 1
                 if (r ^ 1) < n:
                     val[:, 1, r] = mats[:, r, r ^ 1]
@@ -144,6 +289,7 @@ class EllStack:
         self.npad = NP
         self.slots = slots
         self.partner = bool(partner)
+        self.quad = bool(quad)
         self.col = col
         self.val = val
         self.nnz = int(union.sum())
@@ -351,11 +497,14 @@ class Propagator:
     ``mats`` is (K, n, n) (non-rotating) or (S, K, n, n) (rotating frame,
     node-wise generators).  ``weights`` (J, S) and ``nodes`` (S,) are the CF
     tables.  ``eig`` is the (128,) lane factor array of the rotating
-    eigen-step, or None.
+    eigen-step, or None.  ``quad_layout`` (n <= 64) lets the basis order
+    group coupled rows into quads (``quad_order``) whose other three lanes
+    the kernels read with DPP and one 16-byte LDS read, when that saves LDS
+    instructions.
     """
 
     def __init__(self, mats, weights, nodes, rotating=False, eig=None,
-                 permute=None):
+                 permute=None, quad_layout=True):
         torch = _torch()
         _lib.load()
         mats = np.asarray(mats, dtype=_f64)
@@ -389,21 +538,31 @@ class Propagator:
             perm = pair_permutation(np.any(flat != 0, axis=0))
         else:
             perm = np.arange(n0)
+        quad = False
+        pat = np.any(flat != 0, axis=0)[perm][:, perm]
         if n0 > _lib.MAX_N:
             # several wavefronts per instance: widest pairs first, so that
             # each wavefront reads only the slots its own rows use
-            pat = np.any(flat != 0, axis=0)[perm][:, perm]
             perm = perm[width_order(pat)]
-            if eig is not None:
-                eig = np.asarray(eig, dtype=_f64)
-                half = eig.size // 2
-                own, part = eig[:half].copy(), eig[half:].copy()
-                own[:n0], part[:n0] = own[perm], part[perm]
-                eig = np.concatenate([own, part])
+        elif quad_layout:
+            # quads of coupled rows (pairs of eigen-blocks in the rotating
+            # frame): slot 1 by DPP, slots 2-3 by one LDS read, when that
+            # takes fewer LDS instructions than the partner-only layout
+            qo = quad_order(pat, pair_units=not permute)
+            plain = EllStack(pat[None].astype(_f64), partner=True)
+            if 1 + quad_gathers(pat[qo][:, qo]) < plain.gather_slots:
+                perm = perm[qo]
+                quad = True
+        if eig is not None and not np.array_equal(perm, np.arange(n0)):
+            eig = np.asarray(eig, dtype=_f64)
+            half = eig.size // 2
+            own, part = eig[:half].copy(), eig[half:].copy()
+            own[:n0], part[:n0] = own[perm], part[perm]
+            eig = np.concatenate([own, part])
         self.perm = perm
         self.inv_perm = np.argsort(perm)
         flat = flat[:, perm][:, :, perm]
-        self.ell = EllStack(flat, partner=True)
+        self.ell = EllStack(flat, partner=True, quad=quad)
         self.n = self.ell.n
         norms = self.ell.norms
         self.gen_norms = norms.reshape(self.S, self.K) if self.rotating \

@@ -18,6 +18,7 @@ from superspinsim_amd.generate_generators import make_rotating
 from superspinsim_amd.generate_simulator import _flatten, _unflatten, tabulate
 from superspinsim_amd.propagator import (
     EllStack, coefficient_samples, eig_lane_factors, pair_permutation,
+    quad_gathers, quad_order,
     rotating_tables, taylor_plan, taylor_plan_measured, taylor_remainder,
     width_order)
 from superspinsim_amd.sweep import WAVE_DTYPE, Waveform, waveform_records
@@ -57,6 +58,80 @@ def test_pair_permutation_pairs_coherences():
                  if tuple(vm[k][:2]) == tuple(vm[k + 1][:2])
                  and vm[k][2] != vm[k + 1][2])
     assert paired == 21  # all off-diagonal (y, x) of the 7-level model
+
+
+def _outside_quad_bruteforce(pat):
+    """Per row: off-diagonal columns whose lane is not r ^ 1, r ^ 2, r ^ 3."""
+    n = pat.shape[0]
+    return np.array([sum(1 for c in range(n) if pat[r, c] and c != r
+                         and c not in (r ^ 1, r ^ 2, r ^ 3))
+                     for r in range(n)])
+
+
+def _odmr7_pattern(use_incoherent):
+    wl = configs.odmr7(4, use_incoherent=use_incoherent)
+    sim = wl.simulator()
+    active, _, _ = sim.program(wl.waveforms())
+    return np.asarray(wl.generators)[active]
+
+
+@pytest.mark.parametrize("use_incoherent", [False, True])
+def test_quad_order_leaves_three_gathers_on_nv7(use_incoherent):
+    """The 7-level sweep's pattern (n = 49 / 19): the partner-only layout
+    gathers 5 columns per row through LDS; the quad order leaves 3 (rows
+    may land anywhere), 4 when (2k, 2k + 1) pairs must stay together."""
+    mats = _odmr7_pattern(use_incoherent)
+    pat = np.any(mats != 0, axis=0)
+    n = pat.shape[0]
+    perm = pair_permutation(pat)
+    pp = pat[perm][:, perm]
+    assert EllStack(mats[:, perm][:, :, perm]).gather_slots == 5
+    for pair_units, expect in ((False, 3), (True, 4)):
+        order = quad_order(pp, pair_units=pair_units)
+        assert sorted(order) == list(range(n))
+        qp = pp[order][:, order]
+        assert quad_gathers(qp) == expect
+        assert _outside_quad_bruteforce(qp).max() == expect
+        if pair_units:
+            assert np.all(order[0:2 * (n // 2):2] % 2 == 0)
+            assert np.all(order[1:2 * (n // 2):2] ==
+                          order[0:2 * (n // 2):2] + 1)
+            if n % 2:
+                assert order[-1] == n - 1
+    # deterministic: same pattern, same order
+    assert np.array_equal(quad_order(pp), quad_order(pp))
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 9, 30, 64])
+def test_quad_ell_layout(n):
+    """Quad layout: slots 1..3 are r ^ 1 and the other pair of the quad
+    (even row, odd row) on every padded row, the other columns follow, and
+    the table reproduces the matrices exactly."""
+    rng = np.random.default_rng(n)
+    pat = (rng.random((n, n)) < 0.15) | np.eye(n, dtype=bool)
+    mats = np.where(pat[None], rng.standard_normal((2, n, n)), 0.0)
+    order = quad_order(pat)
+    assert sorted(order) == list(range(n))
+    pm = mats[:, order][:, :, order]
+    qp = pat[order][:, order]
+    ell = EllStack(pm, quad=True)
+    assert ell.quad and ell.slots == 4 + quad_gathers(qp)
+    lanes = np.arange(64)
+    other = (lanes & ~1) ^ 2
+    for k, c in enumerate((lanes ^ 1, other, other | 1), start=1):
+        assert np.array_equal(ell.col[k], c)
+    # the three columns are the other members of the lane's quad
+    assert np.all(np.sort(ell.col[1:4], axis=0) ==
+                  np.sort(np.stack([lanes ^ 1, lanes ^ 2, lanes ^ 3]),
+                          axis=0))
+    assert np.array_equal(_dense_from_ell(ell), pm)
+    assert np.all(ell.val[:, :, n:] == 0)
+    assert quad_gathers(qp) == _outside_quad_bruteforce(qp).max()
+    if n > 4:
+        # never worse than the order it started from
+        assert quad_gathers(qp) <= quad_gathers(pat)
+    with pytest.raises(ValueError):
+        EllStack(pm, partner=False, quad=True)
 
 
 @pytest.mark.parametrize("n", [9, 10, 171])
