@@ -209,11 +209,11 @@ class GradientProblem:
         amplitudes = np.asarray(amplitudes, dtype=_f64).ravel()
         gens = np.asarray(self.generators(theta), dtype=_f64)
         waves = self.waveforms(amplitudes)
-        # plain layout: the gradient kernel reads slots 1.. of the shared
-        # forward / transposed pattern through LDS, and the quad order would
-        # widen that pattern
+        # plain layout: the gradient kernel gathers the shared forward /
+        # transposed pattern one column at a time, and the fetch layouts
+        # would widen that pattern
         sim = BatchSimulator(gens, self.J, self.F, tolerance=self.tolerance,
-                             quad_layout=False)
+                             layout="plain")
         active, const, program = sim.program(waves)
         prop = sim._propagator(active, const, self.dt)
         h = self.dt / self.F

@@ -85,71 +85,77 @@ __device__ __forceinline__ void wave_exchange()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The quad layout: slot 1 of row r is column r ^ 1, slots 2 and 3 are the
-// even and the odd row of the other pair of r's quad, (r & ~1) ^ 2 and
-// (r & ~1) ^ 3.  True (wave-uniform) when every lane's columns follow it;
-// slot 1 is then read by DPP and slots 2, 3 by one 16-byte LDS read.
-template <int SLOTS>
-__device__ __forceinline__ bool quad_layout(const int (&col)[SLOTS],
-                                            int lane, bool partner)
+// Padding column of an unused slot s of lane r: r itself for even s, its
+// partner for odd s (the row's own aligned pair); the slot's value is 0.
+__device__ __forceinline__ int pad_column(int s, int lane)
 {
-    if constexpr (SLOTS < 4) {
-        return false;
+    return (s & 1) ? (lane | 1) : (lane & ~1);
+}
+
+// Number of leading slot pairs (2 + 2k, 3 + 2k) that hold the two columns
+// 2p, 2p + 1 of an aligned pair in every lane's row, when slot 1 is the
+// partner r ^ 1 (else 0); wave-uniform.  Such slot pairs are read with one
+// 16-byte LDS read each.
+template <int SLOTS>
+__device__ __forceinline__ int fetched_pairs(const int (&col)[SLOTS],
+                                             int lane, bool partner)
+{
+    int mine = 0;
+    if (partner && SLOTS > 1 && col[1] == (lane ^ 1)) {
+#pragma unroll
+        for (int s = 2; s + 1 < SLOTS; s += 2) {
+            if (mine == (s - 2) / 2 && (col[s] & 1) == 0 &&
+                col[s + 1] == col[s] + 1)
+                ++mine;
+        }
     } else {
-        const int other = (lane & ~1) ^ 2;
-        const bool mine = col[1] == (lane ^ 1) && col[2] == other &&
-                          col[3] == (other | 1);
-        return partner &&
-               __builtin_amdgcn_readfirstlane(__all(mine ? 1 : 0)) != 0;
+        return 0;
     }
+    // the wavefront's minimum
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(mine, off, kWave);
+        mine = o < mine ? o : mine;
+    }
+    return __builtin_amdgcn_readfirstlane(mine);
 }
 
 // y = A v with lane r holding ELL row r of A (values a[], columns col[]) and
 // component r of v: one Taylor product.  Slot 0 is the lane's own
-// component.  With QUAD (the quad layout), slot 1 comes from lane r ^ 1 by
-// DPP and slots 2, 3 (the other pair of the quad) by one 16-byte LDS read;
-// otherwise slot 1 is lane ^ 1 by DPP when `partner` is set.  The other
-// slots are gathered from `xbuf` (one double per lane of the wavefront,
-// 16-byte aligned) after every lane has written its component;
-// LDS instructions of a wavefront execute in order, so a wavefront-scope
-// fence is the only barrier.  Even and odd slots are summed in two chains in
-// slot order, so the rounding depends on the row's values and columns only,
-// not on how the components were fetched.
-template <int SLOTS, bool QUAD>
+// component.  Slot 1 is lane r ^ 1 by DPP when `partner` is set (always
+// when NPAIR > 0).  The first NPAIR slot pairs (2 + 2k, 3 + 2k) hold aligned
+// column pairs (fetched_pairs) and come from one 16-byte LDS read each; the
+// other slots are gathered one by one.  The LDS reads see `xbuf` (one
+// double per lane of the wavefront, 16-byte aligned) after every lane has
+// written its component; LDS instructions of a wavefront execute in order,
+// so a wavefront-scope fence is the only barrier.  Even and odd slots are
+// summed in two chains in slot order, so the rounding depends on the row's
+// values and columns only, not on how the components were fetched.
+template <int SLOTS, int NPAIR>
 __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
                                             const int (&col)[SLOTS],
                                             double v, double *xbuf, int lane,
                                             bool partner)
 {
+    static_assert(NPAIR >= 0 && 2 + 2 * NPAIR <= (SLOTS > 2 ? SLOTS : 2),
+                  "fetched pairs exceed the slots");
     double g[SLOTS];
     g[0] = v;
-    if constexpr (QUAD) {
-        // One LDS instruction for the two slots instead of two gathers.
-        // Reading them by DPP (two moves per double) instead was measured
-        // no faster than the gathers: the moves cost as much issue as the
-        // LDS instructions they remove.
-        static_assert(SLOTS >= 4, "a quad row has at least 4 slots");
-        g[1] = swap_pair(v);
-        xbuf[lane] = v;
-        wave_exchange();
-        const double2 o =
-            reinterpret_cast<const double2 *>(xbuf)[(lane >> 1) ^ 1];
-        g[2] = o.x;
-        g[3] = o.y;
+    xbuf[lane] = v;
+    wave_exchange();
+    if (SLOTS > 1)
+        g[1] = (NPAIR > 0 || partner) ? swap_pair(v) : xbuf[col[1]];
+    const double2 *x2 = reinterpret_cast<const double2 *>(xbuf);
 #pragma unroll
-        for (int sl = 4; sl < SLOTS; ++sl)
-            g[sl] = xbuf[col[sl]];
-        wave_exchange();
-    } else {
-        xbuf[lane] = v;
-        wave_exchange();
-        if (SLOTS > 1)
-            g[1] = partner ? swap_pair(v) : xbuf[col[1]];
-#pragma unroll
-        for (int sl = 2; sl < SLOTS; ++sl)
-            g[sl] = xbuf[col[sl]];
-        wave_exchange();
+    for (int k = 0; k < NPAIR; ++k) {
+        const double2 o = x2[col[2 + 2 * k] >> 1];
+        g[2 + 2 * k] = o.x;
+        g[3 + 2 * k] = o.y;
     }
+#pragma unroll
+    for (int sl = 2 + 2 * NPAIR; sl < SLOTS; ++sl)
+        g[sl] = xbuf[col[sl]];
+    wave_exchange();
     double y0 = a[0] * g[0];
     double y1 = 0.0;
 #pragma unroll

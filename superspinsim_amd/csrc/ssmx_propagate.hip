@@ -21,15 +21,16 @@
 //  - slot 0 of every ELL row is the diagonal: no exchange;
 //  - slot 1 is the partner column r ^ 1 (the Re/Im pair of a coherence, or
 //    a 2x2 eigen-block): exchanged with a DPP quad permutation (VALU);
-//  - in the quad basis order (slots 1..3 of every row are the other lanes
-//    of the row's quad) slots 2 and 3, the other pair of the quad, come
-//    from one 16-byte LDS read (ssmx::ell_apply); the kernel recognises
-//    that layout from the column table;
 //  - slots 2.. are arbitrary columns: each lane writes its component to a
 //    per-wavefront LDS buffer once, then reads its columns with ds_read_b64
 //    (LDS instructions of one wavefront execute in order, so no barrier is
 //    needed; half as many LDS instructions as two 32-bit ds_bpermute per
-//    double).
+//    double);
+//  - in the pair-fetch layout (slots 2 + 2k, 3 + 2k of every row are the
+//    two columns of an aligned pair 2p, 2p + 1, chosen by the host's basis
+//    order) each slot pair comes from one 16-byte LDS read instead
+//    (ssmx::ell_apply); the kernel recognises that layout from the column
+//    table.
 // Coefficients are evaluated lane-parallel in batches: lane e of a batch
 // computes the scalar factor of (step, exponential j, matrix m); the factors
 // are broadcast to the wavefront with readlane when the step runs.
@@ -43,7 +44,8 @@ namespace {
 using ssmx::ell_apply;
 using ssmx::kWave;
 using ssmx::node_time;
-using ssmx::quad_layout;
+using ssmx::pad_column;
+using ssmx::fetched_pairs;
 using ssmx::read_lane;
 using ssmx::set_error;
 using ssmx::swap_pair;
@@ -120,8 +122,8 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
 
 // One instance on one wavefront, after the kernel has staged the generator
 // values (`vals`, [m][slot / 2][lane][slot % 2]) and loaded the lane's ELL
-// columns.  QUAD: slots 1..3 are the other lanes of the quad (ell_apply).
-template <int SLOTS, bool QUAD>
+// columns.  NPAIR: leading slot pairs fetched whole (ell_apply).
+template <int SLOTS, int NPAIR>
 __device__ __forceinline__ void propagate_instance(const Params &p,
                                                    const double *vals,
                                                    double *xbuf, int M,
@@ -245,7 +247,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                         // no contraction: the adjoint's recomputation
                         // repeats these two roundings
 #pragma clang fp contract(off)
-                        term = ell_apply<SLOTS, QUAD>(a, col, term, xbuf,
+                        term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
                                                       lane, use_partner) *
                                read_lane(inv_lane, t);
                         acc += term;
@@ -332,14 +334,26 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
     col[0] = lane;
 #pragma unroll
     for (int s = 1; s < SLOTS; ++s)
-        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
+        col[s] = s < d.slots ? p.ell_col[s * kWave + lane]
+                             : pad_column(s, lane);
+    // slot pairs read whole: every one (rows laid out by pairs), the first
+    // (the other pair of a row's quad) or none; other counts take the
+    // largest variant below them, which reads the same values
+    const int np = fetched_pairs(col, lane, d.partner != 0);
     if constexpr (SLOTS >= 4) {
-        if (quad_layout(col, lane, d.partner != 0)) {
-            propagate_instance<SLOTS, true>(p, lds, xbuf, M, inst, col);
+        constexpr int kAll = (SLOTS - 2) / 2;
+        if constexpr (kAll > 1 && SLOTS <= 16) {
+            if (np >= kAll) {
+                propagate_instance<SLOTS, kAll>(p, lds, xbuf, M, inst, col);
+                return;
+            }
+        }
+        if (np >= 1) {
+            propagate_instance<SLOTS, 1>(p, lds, xbuf, M, inst, col);
             return;
         }
     }
-    propagate_instance<SLOTS, false>(p, lds, xbuf, M, inst, col);
+    propagate_instance<SLOTS, 0>(p, lds, xbuf, M, inst, col);
 }
 
 template <int SLOTS>

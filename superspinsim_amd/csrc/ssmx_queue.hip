@@ -21,7 +21,8 @@ namespace {
 using ssmx::ell_apply;
 using ssmx::kWave;
 using ssmx::node_time;
-using ssmx::quad_layout;
+using ssmx::pad_column;
+using ssmx::fetched_pairs;
 using ssmx::read_lane;
 using ssmx::set_error;
 using ssmx::swap_pair;
@@ -85,9 +86,9 @@ __device__ double step_factor_q(const QParams &p, const ssmx_wave *wv,
 }
 
 // The queue loop of one wavefront, after the kernel has staged the
-// generator values (`vals`) and loaded the lane's ELL columns.  QUAD: slots
-// 1..3 are the other lanes of the quad (ell_apply).
-template <int SLOTS, bool QUAD>
+// generator values (`vals`) and loaded the lane's ELL columns.  NPAIR: leading
+// slot pairs fetched whole (ell_apply).
+template <int SLOTS, int NPAIR>
 __device__ __forceinline__ void queue_loop(const QParams &p,
                                            const double *vals, double *xbuf,
                                            int M, const int (&col)[SLOTS])
@@ -173,7 +174,7 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
                     double acc = x;
                     for (int t = 1; t <= d.taylor; ++t) {
 #pragma clang fp contract(off)
-                        term = ell_apply<SLOTS, QUAD>(a, col, term, xbuf,
+                        term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
                                                       lane, use_partner) *
                                read_lane(inv_lane, t);
                         acc += term;
@@ -231,14 +232,26 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
     col[0] = lane;
 #pragma unroll
     for (int s = 1; s < SLOTS; ++s)
-        col[s] = s < d.slots ? p.ell_col[s * kWave + lane] : lane;
+        col[s] = s < d.slots ? p.ell_col[s * kWave + lane]
+                             : pad_column(s, lane);
+    // slot pairs read whole: every one (rows laid out by pairs), the first
+    // (the other pair of a row's quad) or none; other counts take the
+    // largest variant below them, which reads the same values
+    const int np = fetched_pairs(col, lane, use_partner);
     if constexpr (SLOTS >= 4) {
-        if (quad_layout(col, lane, use_partner)) {
-            queue_loop<SLOTS, true>(p, lds, xbuf, M, col);
+        constexpr int kAll = (SLOTS - 2) / 2;
+        if constexpr (kAll > 1 && SLOTS <= 16) {
+            if (np >= kAll) {
+                queue_loop<SLOTS, kAll>(p, lds, xbuf, M, col);
+                return;
+            }
+        }
+        if (np >= 1) {
+            queue_loop<SLOTS, 1>(p, lds, xbuf, M, col);
             return;
         }
     }
-    queue_loop<SLOTS, false>(p, lds, xbuf, M, col);
+    queue_loop<SLOTS, 0>(p, lds, xbuf, M, col);
 }
 
 template <int SLOTS>

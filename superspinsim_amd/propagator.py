@@ -96,17 +96,37 @@ def width_order(pattern, partner=True):
     return order.astype(np.int64)
 
 
+def _off_diagonal(pattern):
+    need = np.asarray(pattern, dtype=bool).copy()
+    np.fill_diagonal(need, False)
+    return need
+
+
+def fetch_pairs(pattern):
+    """Per row, the aligned pairs (2p, 2p + 1) other than its own that hold
+    one of its off-diagonal columns, ascending.
+
+    In the pair-fetch layout each such pair costs one 16-byte LDS read per
+    product (both columns at once); the row's own pair is its partner
+    r ^ 1, read by DPP.
+    """
+    need = _off_diagonal(pattern)
+    n = need.shape[0]
+    return [sorted({int(c) >> 1 for c in np.nonzero(need[r])[0]}
+                   - {r >> 1}) for r in range(n)]
+
+
+def pair_fetches(pattern):
+    """Largest number of fetched pairs over the rows (``fetch_pairs``)."""
+    pairs = fetch_pairs(pattern)
+    return max((len(p) for p in pairs), default=0)
+
+
 def quad_columns(r):
     """Columns of slots 1, 2, 3 of row r in the quad layout: r ^ 1, then
     the even and the odd row of the other pair of r's quad."""
     other = (r & ~1) ^ 2
     return (r ^ 1, other, other | 1)
-
-
-def _quad_needs(pattern):
-    need = np.asarray(pattern, dtype=bool).copy()
-    np.fill_diagonal(need, False)
-    return need
 
 
 def quad_gathers(pattern):
@@ -115,7 +135,7 @@ def quad_gathers(pattern):
     These are the single LDS gathers per product left in the quad layout,
     beside the write and the 16-byte read of the quad's other pair.
     """
-    need = _quad_needs(pattern)
+    need = _off_diagonal(pattern)
     n = need.shape[0]
     grp = np.arange(n) // 4
     outside = need & (grp[:, None] != grp[None, :])
@@ -137,7 +157,7 @@ def quad_order(pattern, pair_units=False, restarts=8, seed=0):
     last quad is the partial one (an odd last row stays last).  Returns
     ``order`` with ``new[k] = old[order[k]]``.
     """
-    need = _quad_needs(pattern).astype(np.int64)
+    need = _off_diagonal(pattern).astype(np.int64)
     n = need.shape[0]
     if n <= 4:
         return np.arange(n, dtype=np.int64)
@@ -228,20 +248,216 @@ def quad_order(pattern, pair_units=False, restarts=8, seed=0):
     return np.array(order, dtype=np.int64)
 
 
+_PAIR_FETCH_CACHE = {}
+
+
+def pair_fetch_order(pattern, attempts=4, seed=0):
+    """Basis order that minimises the pairs each row fetches.
+
+    Rows are placed at positions 0..n-1; positions (2p, 2p + 1) form pair p.
+    A row reads its partner by DPP and every other pair holding one of its
+    columns with one 16-byte LDS read, so the wavefront executes
+    1 + max_r(pairs of row r) LDS instructions per product (with the write
+    of its own component).  Search: steepest descent over swaps of two rows
+    in different pairs on (max, rows at max, sum), then simulated annealing
+    of the excess over a target one below the best maximum (up to
+    ``attempts`` seeded runs per target), lowering the target until it
+    fails or reaches the bound max_r ceil((d_r - 1) / 2) for d_r
+    off-diagonal columns (the partner is free, a pair holds two).
+    Deterministic for a given seed; cached per pattern.  Returns ``order``
+    with ``new[k] = old[order[k]]``.
+    """
+    need = _off_diagonal(pattern).astype(np.int64)
+    n = need.shape[0]
+    if n <= 2:
+        return np.arange(n, dtype=np.int64)
+    key = (n, np.packbits(need.astype(bool)).tobytes(), attempts, seed)
+    if key in _PAIR_FETCH_CACHE:
+        return _PAIR_FETCH_CACHE[key].copy()
+    npairs = (n + 1) // 2
+    rows = np.arange(n)
+    pair_ind = np.eye(npairs, dtype=np.int64)
+
+    def state(pos):
+        pr = pos >> 1
+        Mc = need @ pair_ind[pr]                  # columns of row r in pair p
+        return Mc, pr, (Mc > 0).sum(axis=1) - (Mc[rows, pr] > 0)
+
+    # steepest descent on (max, rows at max, sum) over all swaps
+    ii, jj = np.triu_indices(n, 1)
+    pos = np.arange(n)
+    while True:
+        Mc, pr, cost = state(pos)
+        m = int(cost.max())
+        cur = (m, int((cost == m).sum()), int(cost.sum()))
+        ok = pr[ii] != pr[jj]
+        I, J = ii[ok], jj[ok]
+        pi, pj = pr[I], pr[J]
+        Mi = Mc[:, pi].T - need[:, I].T + need[:, J].T         # (P, n)
+        Mj = Mc[:, pj].T - need[:, J].T + need[:, I].T
+        D = (Mc > 0).sum(axis=1)[None, :] - (Mc[:, pi].T > 0) \
+            - (Mc[:, pj].T > 0) + (Mi > 0) + (Mj > 0)
+        prn = np.broadcast_to(pr, (I.size, n)).copy()
+        k = np.arange(I.size)
+        prn[k, I] = pj
+        prn[k, J] = pi
+        Mown = np.where(prn == pi[:, None], Mi,
+                        np.where(prn == pj[:, None], Mj,
+                                 Mc[rows, pr][None, :]))
+        C = D - (Mown > 0)
+        mx = C.max(axis=1)
+        cnt = (C == mx[:, None]).sum(axis=1)
+        tot = C.sum(axis=1)
+        b = int(np.lexsort((tot, cnt, mx))[0])
+        if (int(mx[b]), int(cnt[b]), int(tot[b])) >= cur:
+            break
+        pos[I[b]], pos[J[b]] = pos[J[b]], pos[I[b]]
+
+    def anneal(start, target, rs, steps, temp0=0.3, batch=32):
+        # Metropolis over random swaps, `batch` candidates evaluated at once
+        # against the current order; the first accepted one is applied and
+        # the candidates examined count as steps
+        pos = start.copy()
+        Mc, pr, cost = state(pos)
+        Mc = Mc.copy()
+        dist = (Mc > 0).sum(axis=1)               # distinct pairs per row
+        own = Mc[rows, pr] > 0
+        E = float(np.maximum(cost - target, 0).sum()) + 0.01 * cost.sum()
+        k = np.arange(batch)
+        st = 0
+        while st < steps:
+            temp = temp0 * (1.0 - st / steps) + 1e-3
+            I = rs.integers(n, size=batch)
+            J = rs.integers(n, size=batch)
+            u = rs.random(batch)
+            pi, pj = pr[I], pr[J]
+            # only pairs pi and pj change: their column counts, and the
+            # own pair of the rows now in them
+            ci = Mc[:, pi].T + need[:, J].T - need[:, I].T     # (batch, n)
+            cj = Mc[:, pj].T + need[:, I].T - need[:, J].T
+            dist2 = dist[None, :] - (Mc[:, pi].T > 0) \
+                - (Mc[:, pj].T > 0) + (ci > 0) + (cj > 0)
+            pr2 = np.broadcast_to(pr, (batch, n)).copy()
+            pr2[k, I] = pj
+            pr2[k, J] = pi
+            own2 = np.where(pr2 == pi[:, None], ci > 0,
+                            np.where(pr2 == pj[:, None], cj > 0,
+                                     own[None, :]))
+            c2 = dist2 - own2
+            E2 = np.maximum(c2 - target, 0).sum(axis=1) + 0.01 * c2.sum(axis=1)
+            ok = (pi != pj) & ((E2 <= E) |
+                               (u < np.exp(np.minimum(E - E2, 0.0) / temp)))
+            hits = np.flatnonzero(ok)
+            if hits.size == 0:
+                st += batch
+                continue
+            b = int(hits[0])
+            st += b + 1
+            Mc[:, pi[b]], Mc[:, pj[b]] = ci[b], cj[b]
+            pr, dist, own = pr2[b], dist2[b], own2[b]
+            cost, E = c2[b], float(E2[b])
+            a, c = int(I[b]), int(J[b])
+            pos[a], pos[c] = pos[c], pos[a]
+            if cost.max() <= target:
+                return pos
+        return None
+
+    degree = need.sum(axis=1)
+    bound = int(np.max(degree // 2))      # ceil((d - 1) / 2)
+    best = pos
+    T = int(state(best)[2].max())
+    rs = np.random.default_rng(seed)
+    while T > bound:
+        found = None
+        for _ in range(attempts):
+            found = anneal(best, T - 1, rs, 1500 * n)
+            if found is not None:
+                break
+        if found is None:
+            break
+        best = found
+        T -= 1
+    order = np.empty(n, dtype=np.int64)
+    order[best] = np.arange(n)
+    _PAIR_FETCH_CACHE[key] = order.copy()
+    return order
+
+
+def _bank_slots(entries, G, pad, group, banks):
+    """Order each row's fetch addresses over G slots so that, slot by slot,
+    the lanes of one LDS cycle (``group`` consecutive lanes) read few
+    distinct addresses that share a bank (``address % banks``).
+
+    ``entries[r]`` lists row r's addresses (any order, at most G); slots
+    left over take an address another lane of the group reads in that slot
+    (a broadcast), else ``pad(r)``.  Local search by swapping two slots of
+    one row, group by group, while the summed passes drop.  Returns an
+    (R, G) integer array.
+    """
+    R = len(entries)
+    out = np.zeros((R, G), dtype=np.int64)
+
+    def slot_passes(cur, k):
+        per = {}
+        for a in {sl[k] for sl in cur if sl[k] is not None}:
+            per[a % banks] = per.get(a % banks, 0) + 1
+        return max(per.values(), default=0)
+
+    for g0 in range(0, R, group):
+        rows = list(range(g0, min(g0 + group, R)))
+        cur = [list(entries[r]) + [None] * (G - len(entries[r]))
+               for r in rows]
+        cost = [slot_passes(cur, k) for k in range(G)]
+        improved = True
+        while improved:
+            improved = False
+            for i in range(len(rows)):
+                for k1 in range(G):
+                    for k2 in range(k1 + 1, G):
+                        a, b = cur[i][k1], cur[i][k2]
+                        if a == b:
+                            continue
+                        cur[i][k1], cur[i][k2] = b, a
+                        c1, c2 = slot_passes(cur, k1), slot_passes(cur, k2)
+                        if c1 + c2 < cost[k1] + cost[k2]:
+                            cost[k1], cost[k2] = c1, c2
+                            improved = True
+                        else:
+                            cur[i][k1], cur[i][k2] = a, b
+        for i, r in enumerate(rows):
+            for k in range(G):
+                a = cur[i][k]
+                if a is None:
+                    used = [cur[q][k] for q in range(len(rows))
+                            if cur[q][k] is not None]
+                    a = used[0] if used else pad(r)
+                out[r, k] = a
+    return out
+
+
 class EllStack:
     """Generators packed in one ELL pattern (the union of their non-zeros).
 
     ``mats`` has shape (M, n, n).  Slot 0 of row r is always column r (the
-    diagonal); with ``partner`` slot 1 is always column r ^ 1; with ``quad``
-    (n <= 64) slots 1, 2, 3 are always the other rows of r's quad,
-    ``quad_columns(r)``, for every row of the padded table (the kernels
-    recognise this layout and read those slots without LDS gathers).  The other non-zeros
-    of the union fill the next slots in column order.  ``col`` is int32
+    diagonal); with ``partner`` slot 1 is always column r ^ 1.  ``layout``
+    places the other non-zeros of the union:
+
+    * ``"plain"``: slots 2.. in column order, gathered one by one;
+    * ``"quad"`` (n <= 64): slots 2, 3 are the other pair of r's quad
+      (``quad_columns``), the columns outside the quad follow;
+    * ``"pairs"`` (n <= 64): slots 2 + 2k, 3 + 2k are the two columns
+      2p, 2p + 1 of the row's k-th fetched pair (``fetch_pairs``).
+
+    In the last two every row of the padded table follows the layout, and
+    the kernels read each such slot pair with one 16-byte LDS read.  With
+    ``bank_aware`` the fetched addresses are ordered over the slots so that
+    lanes read by one LDS cycle meet fewer bank conflicts.  ``col`` is int32
     (slots, NP), ``val`` float64 (M, slots, NP) with NP = 64 ceil(n / 64);
-    unused slots point at column r with value 0, rows >= n are all zero.
+    unused slots point at the row's own pair with value 0, rows >= n are
+    all zero.
     """
 
-    def __init__(self, mats, partner=True, quad=False):
+    def __init__(self, mats, partner=True, layout="plain", bank_aware=False):
         mats = np.asarray(mats, dtype=_f64)
         if mats.ndim != 3 or mats.shape[1] != mats.shape[2]:
             raise ValueError(f"generator stack must be (M, n, n), got "
@@ -250,50 +466,87 @@ class EllStack:
         if n > _lib.MAX_N_WIDE:
             raise ValueError(f"superoperator dimension {n} > "
                              f"{_lib.MAX_N_WIDE}")
+        if layout not in ("plain", "quad", "pairs"):
+            raise ValueError(f"unknown ELL layout {layout!r}")
         NP = padded_rows(n)
-        if quad and (not partner or NP != 64):
-            raise ValueError("the quad layout needs partner=True and n <= 64")
+        if layout != "plain" and (not partner or NP != 64):
+            raise ValueError(f"the {layout} layout needs partner=True and "
+                             "n <= 64")
         union = np.any(mats != 0, axis=0)
-        fixed = 4 if quad else (2 if partner else 1)
-        rows = []
-        for r in range(n):
-            special = {r, *quad_columns(r)} if quad else \
-                ({r} | ({r ^ 1} if partner else set()))
-            rows.append([c for c in np.nonzero(union[r])[0]
-                         if c not in special])
-        slots = fixed + max(len(c) for c in rows)
-        col = np.tile(np.arange(NP, dtype=np.int32), (slots, 1))
+        lanes = np.arange(NP, dtype=np.int64)
+        fixed = 2 if partner else 1
+        if layout == "pairs":
+            pairs = fetch_pairs(union) + [[] for _ in range(NP - n)]
+            G = max((len(p) for p in pairs), default=0)
+            if bank_aware:
+                fp = _bank_slots(pairs, G, lambda r: r >> 1, 8, 8)
+            else:
+                fp = np.array([p + [r >> 1] * (G - len(p))
+                               for r, p in enumerate(pairs)],
+                              dtype=np.int64).reshape(NP, G)
+            slot_cols = np.empty((NP, 2 * G), dtype=np.int64)
+            slot_cols[:, 0::2] = 2 * fp
+            slot_cols[:, 1::2] = 2 * fp + 1
+            used = [fixed + 2 * len(p) for p in pairs[:n]]
+        else:
+            head = []
+            singles = []
+            for r in range(NP):
+                special = {r, r ^ 1} if partner else {r}
+                if layout == "quad":
+                    q = quad_columns(r)
+                    special |= set(q)
+                    head.append([q[1], q[2]])
+                cols = [] if r >= n else \
+                    [int(c) for c in np.nonzero(union[r])[0]
+                     if c not in special]
+                singles.append(cols)
+            S = max((len(c) for c in singles[:n]), default=0)
+            if bank_aware and layout == "quad":
+                sc = _bank_slots(singles, S, lambda r: r, 16, 16)
+            else:
+                sc = np.array([c + [r] * (S - len(c))
+                               for r, c in enumerate(singles)],
+                              dtype=np.int64).reshape(NP, S)
+            slot_cols = np.concatenate([np.array(head, dtype=np.int64)
+                                        .reshape(NP, -1), sc], axis=1) \
+                if layout == "quad" else sc
+            used = [fixed + (2 if layout == "quad" else 0) + len(c)
+                    for c in singles[:n]]
+        slots = fixed + slot_cols.shape[1]
+        col = np.empty((slots, NP), dtype=np.int32)
+        col[0] = lanes
+        if partner:
+            col[1] = lanes ^ 1
+        col[fixed:] = slot_cols.T
         val = np.zeros((M, slots, NP), dtype=_f64)
-        if quad:
-            lanes = np.arange(NP, dtype=np.int32)
-            col[1:4, :] = quad_columns(lanes)
         for r in range(n):
             val[:, 0, r] = mats[:, r, r]
-            if quad:
-                for k, c in enumerate(quad_columns(r), start=1):
-                    if c < n:
-                        val[:, k, r] = mats[:, r, c]
-            elif partner:
-                col[1, r] = r ^ 1
-                if (r ^ 1) < n:
-                    val[:, 1, r] = mats[:, r, r ^ 1]
-            for k, c in enumerate(rows[r]):
-                col[fixed + k, r] = c
-                val[:, fixed + k, r] = mats[:, r, c]
+            if partner and (r ^ 1) < n:
+                val[:, 1, r] = mats[:, r, r ^ 1]
+            seen = {r, r ^ 1} if partner else {r}
+            for sl in range(fixed, slots):
+                c = int(col[sl, r])
+                # each non-zero once: padding repeats read value 0
+                if c < n and c not in seen and union[r, c]:
+                    val[:, sl, r] = mats[:, r, c]
+                    seen.add(c)
         # slot 0 is the diagonal by definition; its column entry carries the
         # number of used slots of each row
         col[0, :] = fixed
-        for r in range(n):
-            col[0, r] = fixed + len(rows[r])
+        col[0, :n] = used
         self.n = n
         self.npad = NP
         self.slots = slots
         self.partner = bool(partner)
-        self.quad = bool(quad)
+        self.layout = layout
         self.col = col
         self.val = val
         self.nnz = int(union.sum())
-        self.gather_slots = slots - fixed
+        #: LDS reads per product beside the write (one per fetched pair)
+        self.gather_slots = {"plain": slots - fixed,
+                             "quad": slots - fixed - 1,
+                             "pairs": (slots - fixed) // 2}[layout]
         self.norms = np.abs(mats).sum(axis=1).max(axis=1)  # 1-norms
 
 
@@ -491,20 +744,63 @@ def eig_lane_factors(d_step, s_step, n, residual=True):
 # Launch
 # ---------------------------------------------------------------------------
 
+def lds_instructions(pattern, layout):
+    """LDS instructions per Taylor product of a row layout on ``pattern``
+    (n <= 64, already in the layout's basis order): the write of the lane's
+    component plus one read per gathered column or fetched pair."""
+    if layout == "plain":
+        return 1 + EllStack(pattern[None].astype(_f64)).gather_slots
+    if layout == "quad":
+        return 2 + quad_gathers(pattern)
+    return 1 + pair_fetches(pattern)
+
+
+def choose_layout(pattern, layout="auto", pair_units=False):
+    """(order, layout) of the ELL table for an n <= 64 pattern.
+
+    ``"plain"`` keeps the order.  ``"quad"`` and ``"pairs"`` return their
+    searched order (``quad_order``, ``pair_fetch_order``; with
+    ``pair_units`` the rows (2k, 2k + 1) stay together, and the pair layout
+    keeps the order).  ``"auto"`` takes the pair layout up to 32 rows and
+    the quad layout above, if it needs fewer LDS instructions per product
+    than the plain one, else the plain one.  Measured on the 7-level NV
+    sweep (4096 instances, same process): n = 49 plain 518, quad 474, pairs
+    490 ms; n = 19 plain 496, quad 460, pairs 445 ms.  With every lane
+    active the pair layout's scattered 16-byte reads meet more bank
+    conflicts; with half the lanes idle its fewer instructions win.
+    """
+    n = pattern.shape[0]
+    ident = np.arange(n, dtype=np.int64)
+    if layout == "plain" or n < 2:
+        return ident, "plain"
+    if layout == "auto":
+        layout = "pairs" if n <= 32 else "quad"
+        order, _ = choose_layout(pattern, layout, pair_units)
+        better = lds_instructions(pattern[order][:, order], layout) < \
+            lds_instructions(pattern, "plain")
+        return (order, layout) if better else (ident, "plain")
+    if layout == "quad":
+        return quad_order(pattern, pair_units=pair_units), "quad"
+    if layout == "pairs":
+        order = ident if pair_units else pair_fetch_order(pattern)
+        return order, "pairs"
+    raise ValueError(f"unknown ELL layout {layout!r}")
+
+
 class Propagator:
     """Device copy of one generator stack and its CF scheme.
 
     ``mats`` is (K, n, n) (non-rotating) or (S, K, n, n) (rotating frame,
     node-wise generators).  ``weights`` (J, S) and ``nodes`` (S,) are the CF
     tables.  ``eig`` is the (128,) lane factor array of the rotating
-    eigen-step, or None.  ``quad_layout`` (n <= 64) lets the basis order
-    group coupled rows into quads (``quad_order``) whose other three lanes
-    the kernels read with DPP and one 16-byte LDS read, when that saves LDS
-    instructions.
+    eigen-step, or None.  ``layout`` (n <= 64) picks the ELL layout and
+    the basis order behind it (``choose_layout``, ``EllStack``);
+    ``bank_aware`` orders fetched addresses over the slots against LDS bank
+    conflicts.
     """
 
     def __init__(self, mats, weights, nodes, rotating=False, eig=None,
-                 permute=None, quad_layout=True):
+                 permute=None, layout="auto", bank_aware=True):
         torch = _torch()
         _lib.load()
         mats = np.asarray(mats, dtype=_f64)
@@ -538,21 +834,16 @@ class Propagator:
             perm = pair_permutation(np.any(flat != 0, axis=0))
         else:
             perm = np.arange(n0)
-        quad = False
+        self.layout = "plain"
         pat = np.any(flat != 0, axis=0)[perm][:, perm]
         if n0 > _lib.MAX_N:
             # several wavefronts per instance: widest pairs first, so that
             # each wavefront reads only the slots its own rows use
             perm = perm[width_order(pat)]
-        elif quad_layout:
-            # quads of coupled rows (pairs of eigen-blocks in the rotating
-            # frame): slot 1 by DPP, slots 2-3 by one LDS read, when that
-            # takes fewer LDS instructions than the partner-only layout
-            qo = quad_order(pat, pair_units=not permute)
-            plain = EllStack(pat[None].astype(_f64), partner=True)
-            if 1 + quad_gathers(pat[qo][:, qo]) < plain.gather_slots:
-                perm = perm[qo]
-                quad = True
+        elif layout != "plain":
+            order, self.layout = choose_layout(pat, layout,
+                                               pair_units=not permute)
+            perm = perm[order]
         if eig is not None and not np.array_equal(perm, np.arange(n0)):
             eig = np.asarray(eig, dtype=_f64)
             half = eig.size // 2
@@ -562,7 +853,8 @@ class Propagator:
         self.perm = perm
         self.inv_perm = np.argsort(perm)
         flat = flat[:, perm][:, :, perm]
-        self.ell = EllStack(flat, partner=True, quad=quad)
+        self.ell = EllStack(flat, partner=True, layout=self.layout,
+                            bank_aware=bank_aware)
         self.n = self.ell.n
         norms = self.ell.norms
         self.gen_norms = norms.reshape(self.S, self.K) if self.rotating \

@@ -153,16 +153,16 @@ class BatchSimulator:
     Parameters mirror ``generate_simulator``: ``generators`` (G, n, n) in the
     order of the coefficient rows, the CF scheme, fine divisions and the
     optional rotating frame (eigen-data from ``make_rotating``; generators
-    then exclude L0).  ``quad_layout`` lets the propagators group coupled
-    rows into quads that take fewer LDS instructions (``Propagator``).
+    then exclude L0).  ``layout`` is the propagators' ELL layout
+    (``Propagator``).
     """
 
     def __init__(self, generators, number_of_exponentials=2,
                  number_of_fine_divisions=1, use_rotating=False,
                  vectors_real=None, inv_vectors_real=None, doubles=None,
-                 singles=None, tolerance=1e-16, quad_layout=True):
+                 singles=None, tolerance=1e-16, layout="auto"):
         self.gens = np.asarray(generators, dtype=_f64)
-        self.quad_layout = bool(quad_layout)
+        self.layout = layout
         self.G, self.n = self.gens.shape[0], self.gens.shape[1]
         self.nodes, self.weights, self.label = quadratures.scheme(
             number_of_exponentials)
@@ -210,7 +210,7 @@ class BatchSimulator:
                 eig = None
             self._prop_cache[key] = Propagator(
                 mats, self.weights, self.nodes, rotating=self.rotating,
-                eig=eig, quad_layout=self.quad_layout)
+                eig=eig, layout=self.layout)
         return self._prop_cache[key]
 
     def program(self, waves):

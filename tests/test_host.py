@@ -18,7 +18,8 @@ from superspinsim_amd.generate_generators import make_rotating
 from superspinsim_amd.generate_simulator import _flatten, _unflatten, tabulate
 from superspinsim_amd.propagator import (
     EllStack, coefficient_samples, eig_lane_factors, pair_permutation,
-    quad_gathers, quad_order,
+    choose_layout, fetch_pairs, lds_instructions, pair_fetch_order,
+    pair_fetches, quad_gathers, quad_order,
     rotating_tables, taylor_plan, taylor_plan_measured, taylor_remainder,
     width_order)
 from superspinsim_amd.sweep import WAVE_DTYPE, Waveform, waveform_records
@@ -60,12 +61,12 @@ def test_pair_permutation_pairs_coherences():
     assert paired == 21  # all off-diagonal (y, x) of the 7-level model
 
 
-def _outside_quad_bruteforce(pat):
-    """Per row: off-diagonal columns whose lane is not r ^ 1, r ^ 2, r ^ 3."""
+def _pairs_bruteforce(pat):
+    """Per row: distinct aligned pairs (c // 2) other than its own holding
+    an off-diagonal column."""
     n = pat.shape[0]
-    return np.array([sum(1 for c in range(n) if pat[r, c] and c != r
-                         and c not in (r ^ 1, r ^ 2, r ^ 3))
-                     for r in range(n)])
+    return np.array([len({c // 2 for c in range(n) if pat[r, c] and c != r}
+                         - {r // 2}) for r in range(n)])
 
 
 def _odmr7_pattern(use_incoherent):
@@ -73,6 +74,66 @@ def _odmr7_pattern(use_incoherent):
     sim = wl.simulator()
     active, _, _ = sim.program(wl.waveforms())
     return np.asarray(wl.generators)[active]
+
+
+@pytest.mark.parametrize("use_incoherent", [False, True])
+def test_pair_fetch_order_on_nv7(use_incoherent):
+    """The 7-level sweep's pattern (n = 49 / 19): gathering column by column
+    takes 5 LDS reads per product; the pair-fetch order leaves at most 3
+    aligned pairs per row (one 16-byte read each)."""
+    mats = _odmr7_pattern(use_incoherent)
+    pat = np.any(mats != 0, axis=0)
+    n = pat.shape[0]
+    perm = pair_permutation(pat)
+    pp = pat[perm][:, perm]
+    assert EllStack(mats[:, perm][:, :, perm]).gather_slots == 5
+    order = pair_fetch_order(pp)
+    assert sorted(order) == list(range(n))
+    qp = pp[order][:, order]
+    assert pair_fetches(qp) == 3
+    assert _pairs_bruteforce(qp).max() == 3
+    assert [len(f) for f in fetch_pairs(qp)] == list(_pairs_bruteforce(qp))
+    ell = EllStack(mats[:, perm[order]][:, :, perm[order]], layout="pairs")
+    assert ell.slots == 8 and ell.gather_slots == 3
+    # deterministic: same pattern, same order
+    assert np.array_equal(pair_fetch_order(pp), order)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 9, 30, 64])
+def test_pair_fetch_ell_layout(n):
+    """Pair-fetch layout: slot 1 is r ^ 1 and every later slot pair is an
+    aligned column pair (2p, 2p + 1) on every padded row; the table
+    reproduces the matrices exactly; the order never fetches more pairs
+    than the order it started from."""
+    rng = np.random.default_rng(n)
+    pat = (rng.random((n, n)) < 0.15) | np.eye(n, dtype=bool)
+    mats = np.where(pat[None], rng.standard_normal((2, n, n)), 0.0)
+    order = pair_fetch_order(pat)
+    assert sorted(order) == list(range(n))
+    pm = mats[:, order][:, :, order]
+    qp = pat[order][:, order]
+    ell = EllStack(pm, layout="pairs")
+    G = pair_fetches(qp)
+    assert ell.layout == "pairs" and ell.slots == 2 + 2 * G
+    assert G == (_pairs_bruteforce(qp).max() if n else 0)
+    lanes = np.arange(64)
+    assert np.array_equal(ell.col[1], lanes ^ 1)
+    for k in range(G):
+        assert np.all(ell.col[2 + 2 * k] % 2 == 0)
+        assert np.array_equal(ell.col[3 + 2 * k], ell.col[2 + 2 * k] + 1)
+    assert np.array_equal(_dense_from_ell(ell), pm)
+    assert np.all(ell.val[:, :, n:] == 0)
+    assert G <= pair_fetches(pat)
+    with pytest.raises(ValueError):
+        EllStack(pm, partner=False, layout="pairs")
+
+
+def _outside_quad_bruteforce(pat):
+    """Per row: off-diagonal columns whose lane is not r ^ 1, r ^ 2, r ^ 3."""
+    n = pat.shape[0]
+    return np.array([sum(1 for c in range(n) if pat[r, c] and c != r
+                         and c not in (r ^ 1, r ^ 2, r ^ 3))
+                     for r in range(n)])
 
 
 @pytest.mark.parametrize("use_incoherent", [False, True])
@@ -114,8 +175,8 @@ def test_quad_ell_layout(n):
     assert sorted(order) == list(range(n))
     pm = mats[:, order][:, :, order]
     qp = pat[order][:, order]
-    ell = EllStack(pm, quad=True)
-    assert ell.quad and ell.slots == 4 + quad_gathers(qp)
+    ell = EllStack(pm, layout="quad")
+    assert ell.layout == "quad" and ell.slots == 4 + quad_gathers(qp)
     lanes = np.arange(64)
     other = (lanes & ~1) ^ 2
     for k, c in enumerate((lanes ^ 1, other, other | 1), start=1):
@@ -131,7 +192,70 @@ def test_quad_ell_layout(n):
         # never worse than the order it started from
         assert quad_gathers(qp) <= quad_gathers(pat)
     with pytest.raises(ValueError):
-        EllStack(pm, partner=False, quad=True)
+        EllStack(pm, partner=False, layout="quad")
+
+
+def _bank_passes(addrs, group, banks):
+    """LDS cycles of one read: per group of lanes, the most distinct
+    addresses that share a bank."""
+    total = 0
+    for g0 in range(0, addrs.size, group):
+        per = {}
+        for a in set(addrs[g0:g0 + group].tolist()):
+            per[a % banks] = per.get(a % banks, 0) + 1
+        total += max(per.values())
+    return total
+
+
+@pytest.mark.parametrize("layout", ["quad", "pairs"])
+@pytest.mark.parametrize("n", [19, 49, 64])
+def test_bank_aware_slot_order(layout, n):
+    """Bank-aware slot order: same layout rules and the same matrices, and
+    no more bank passes slot by slot summed (8-lane groups for the 16-byte
+    reads, 16-lane groups for single gathers)."""
+    rng = np.random.default_rng(n)
+    pat = (rng.random((n, n)) < 0.1) | np.eye(n, dtype=bool)
+    mats = np.where(pat[None], rng.standard_normal((2, n, n)), 0.0)
+    order, _ = choose_layout(pat, layout)
+    pm = mats[:, order][:, :, order]
+    plain_ord = EllStack(pm, layout=layout)
+    aware = EllStack(pm, layout=layout, bank_aware=True)
+    assert aware.slots == plain_ord.slots
+    assert np.array_equal(_dense_from_ell(aware), pm)
+    assert np.all(aware.val[:, :, n:] == 0)
+    lanes = np.arange(64)
+    assert np.array_equal(aware.col[1], lanes ^ 1)
+    npair = 1 if layout == "quad" else (aware.slots - 2) // 2
+    for k in range(npair):
+        assert np.all(aware.col[2 + 2 * k] % 2 == 0)
+        assert np.array_equal(aware.col[3 + 2 * k], aware.col[2 + 2 * k] + 1)
+
+    def passes(ell):
+        c = ell.col.astype(np.int64)
+        tot = sum(_bank_passes(c[2 + 2 * k] >> 1, 8, 8)
+                  for k in range(npair))
+        return tot + sum(_bank_passes(c[sl], 16, 16)
+                         for sl in range(2 + 2 * npair, ell.slots))
+    assert passes(aware) <= passes(plain_ord)
+
+
+@pytest.mark.parametrize("use_incoherent,expect,instr",
+                         [(False, "quad", 5), (True, "pairs", 4)])
+def test_auto_layout_on_nv7(use_incoherent, expect, instr):
+    """The 7-level sweep: quads at n = 49, pairs at n = 19, each below the
+    plain layout's 6 LDS instructions per product."""
+    mats = _odmr7_pattern(use_incoherent)
+    pat = np.any(mats != 0, axis=0)
+    perm = pair_permutation(pat)
+    pp = pat[perm][:, perm]
+    assert lds_instructions(pp, "plain") == 6
+    order, layout = choose_layout(pp)
+    assert layout == expect
+    assert lds_instructions(pp[order][:, order], layout) == instr
+    # the rotating frame keeps its (2k, 2k + 1) blocks together
+    order, layout = choose_layout(pp, pair_units=True)
+    n = pp.shape[0]
+    assert np.all(order[1:2 * (n // 2):2] == order[0:2 * (n // 2):2] + 1)
 
 
 @pytest.mark.parametrize("n", [9, 10, 171])
