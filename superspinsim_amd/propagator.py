@@ -532,7 +532,11 @@ class EllStack:
                     val[:, sl, r] = mats[:, r, c]
                     seen.add(c)
         # slot 0 is the diagonal by definition; its column entry carries the
-        # number of used slots of each row
+        # number of used slots of each row: the row's non-zeros lie in the
+        # slots below it (a bank-aware order may interleave padding, so the
+        # fetch layouts then count every slot)
+        if bank_aware and layout != "plain":
+            used = [slots] * n
         col[0, :] = fixed
         col[0, :n] = used
         self.n = n

@@ -223,6 +223,11 @@ def test_bank_aware_slot_order(layout, n):
     assert aware.slots == plain_ord.slots
     assert np.array_equal(_dense_from_ell(aware), pm)
     assert np.all(aware.val[:, :, n:] == 0)
+    # every non-zero lies below the row's used-slot count
+    used = aware.col[0, :n]
+    for r in range(n):
+        for sl in range(int(used[r]), aware.slots):
+            assert np.all(aware.val[:, sl, r] == 0)
     lanes = np.arange(64)
     assert np.array_equal(aware.col[1], lanes ^ 1)
     npair = 1 if layout == "quad" else (aware.slots - 2) // 2
