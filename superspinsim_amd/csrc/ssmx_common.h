@@ -168,6 +168,46 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
     return y0 + y1;
 }
 
+// exp(A) x by its Taylor series of degree m: term <- (A term) * (1 / t),
+// acc += term for t = 1..m, each product and sum rounded on its own (no
+// contraction), so every kernel that repeats the chain gets the same bits.
+// M > 0: the degree is the compile-time M, the chain is unrolled and 1 / t
+// are constants; M = 0: the runtime degree m, 1 / t broadcast from lane t
+// of inv_lane (which holds 1 / lane).
+template <int SLOTS, int NPAIR, int M>
+__device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
+                                                const int (&col)[SLOTS],
+                                                double x, double *xbuf,
+                                                int lane, bool partner, int m,
+                                                double inv_lane)
+{
+    double term = x;
+    double acc = x;
+    if constexpr (M > 0) {
+#pragma unroll
+        for (int t = 1; t <= M; ++t) {
+#pragma clang fp contract(off)
+            term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
+                                           partner) *
+                   (1.0 / t);
+            acc += term;
+        }
+    } else {
+        for (int t = 1; t <= m; ++t) {
+#pragma clang fp contract(off)
+            term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
+                                           partner) *
+                   read_lane(inv_lane, t);
+            acc += term;
+        }
+    }
+    return acc;
+}
+
+// The Taylor degree with an unrolled chain (the degree the 1e-16
+// tolerance gives at the 20 ps steps of the NV workloads).
+constexpr int kUnrolledTaylor = 11;
+
 // Node time of global output interval i, fine division r, node c, in the
 // reference's order without contraction:
 //   ((t0 + (r dt) / F) + dt i) + (dt c) / F

@@ -42,12 +42,14 @@
 namespace {
 
 using ssmx::ell_apply;
+using ssmx::kUnrolledTaylor;
 using ssmx::kWave;
 using ssmx::node_time;
 using ssmx::pad_column;
 using ssmx::fetched_pairs;
 using ssmx::read_lane;
 using ssmx::set_error;
+using ssmx::taylor_action;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
 using ssmx::wave_sum;
@@ -123,7 +125,7 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
 // One instance on one wavefront, after the kernel has staged the generator
 // values (`vals`, [m][slot / 2][lane][slot % 2]) and loaded the lane's ELL
 // columns.  NPAIR: leading slot pairs fetched whole (ell_apply).
-template <int SLOTS, int NPAIR>
+template <int SLOTS, int NPAIR, int TAY>
 __device__ __forceinline__ void propagate_instance(const Params &p,
                                                    const double *vals,
                                                    double *xbuf, int M,
@@ -240,20 +242,11 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                     for (int sl = 0; sl < SLOTS; ++sl)
                         a[sl] *= inv_sub;
                 }
-                for (int sub = 0; sub < d.substeps; ++sub) {
-                    double term = x;
-                    double acc = x;
-                    for (int t = 1; t <= d.taylor; ++t) {
-                        // no contraction: the adjoint's recomputation
-                        // repeats these two roundings
-#pragma clang fp contract(off)
-                        term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
-                                                      lane, use_partner) *
-                               read_lane(inv_lane, t);
-                        acc += term;
-                    }
-                    x = acc;
-                }
+                // the adjoint's recomputation repeats these roundings
+                for (int sub = 0; sub < d.substeps; ++sub)
+                    x = taylor_action<SLOTS, NPAIR, TAY>(
+                        a, col, x, xbuf, lane, use_partner, d.taylor,
+                        inv_lane);
             }
             if (rotating) {
                 const double xp = swap_pair(x);
@@ -291,8 +284,25 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
         p.state_out[(size_t)inst * n + lane] = x;
 }
 
+// The Taylor chain unrolled for the common degree (SLOTS <= 16), else the
+// runtime loop.
+template <int SLOTS, int NPAIR>
+__device__ __forceinline__ void propagate_instance_any(const Params &p, const double *vals, double *xbuf, int M, int inst, const int (&col)[SLOTS])
+{
+    if constexpr (SLOTS <= 16) {
+        if (p.d.taylor == kUnrolledTaylor) {
+            propagate_instance<SLOTS, NPAIR, kUnrolledTaylor>(p, vals, xbuf, M, inst, col);
+            return;
+        }
+    }
+    propagate_instance<SLOTS, NPAIR, 0>(p, vals, xbuf, M, inst, col);
+}
+
 template <int SLOTS>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
+// At most 128 VGPRs: four wavefronts per SIMD (4096 instances fill the
+// device once).
+__global__ __launch_bounds__(kWave *kWavesPerBlock)
+__attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
     Params p)
 {
     // LDS: generator values staged as [m][slot / 2][lane][slot % 2] (one
@@ -344,16 +354,16 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void propagate_kernel(
         constexpr int kAll = (SLOTS - 2) / 2;
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
-                propagate_instance<SLOTS, kAll>(p, lds, xbuf, M, inst, col);
+                propagate_instance_any<SLOTS, kAll>(p, lds, xbuf, M, inst, col);
                 return;
             }
         }
         if (np >= 1) {
-            propagate_instance<SLOTS, 1>(p, lds, xbuf, M, inst, col);
+            propagate_instance_any<SLOTS, 1>(p, lds, xbuf, M, inst, col);
             return;
         }
     }
-    propagate_instance<SLOTS, 0>(p, lds, xbuf, M, inst, col);
+    propagate_instance_any<SLOTS, 0>(p, lds, xbuf, M, inst, col);
 }
 
 template <int SLOTS>

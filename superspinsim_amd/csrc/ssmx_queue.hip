@@ -19,12 +19,14 @@
 namespace {
 
 using ssmx::ell_apply;
+using ssmx::kUnrolledTaylor;
 using ssmx::kWave;
 using ssmx::node_time;
 using ssmx::pad_column;
 using ssmx::fetched_pairs;
 using ssmx::read_lane;
 using ssmx::set_error;
+using ssmx::taylor_action;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
 using ssmx::wave_sum;
@@ -88,7 +90,7 @@ __device__ double step_factor_q(const QParams &p, const ssmx_wave *wv,
 // The queue loop of one wavefront, after the kernel has staged the
 // generator values (`vals`) and loaded the lane's ELL columns.  NPAIR: leading
 // slot pairs fetched whole (ell_apply).
-template <int SLOTS, int NPAIR>
+template <int SLOTS, int NPAIR, int TAY>
 __device__ __forceinline__ void queue_loop(const QParams &p,
                                            const double *vals, double *xbuf,
                                            int M, const int (&col)[SLOTS])
@@ -169,18 +171,11 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
                     for (int sl = 0; sl < SLOTS; ++sl)
                         a[sl] *= inv_sub;
                 }
-                for (int sub = 0; sub < d.substeps; ++sub) {
-                    double term = x;
-                    double acc = x;
-                    for (int t = 1; t <= d.taylor; ++t) {
-#pragma clang fp contract(off)
-                        term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
-                                                      lane, use_partner) *
-                               read_lane(inv_lane, t);
-                        acc += term;
-                    }
-                    x = acc;
-                }
+                // the adjoint's recomputation repeats these roundings
+                for (int sub = 0; sub < d.substeps; ++sub)
+                    x = taylor_action<SLOTS, NPAIR, TAY>(
+                        a, col, x, xbuf, lane, use_partner, d.taylor,
+                        inv_lane);
             }
             if (rotating) {
                 const double xp = swap_pair(x);
@@ -200,8 +195,24 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
     }
 }
 
+// The Taylor chain unrolled for the common degree (SLOTS <= 16), else the
+// runtime loop.
+template <int SLOTS, int NPAIR>
+__device__ __forceinline__ void queue_loop_any(const QParams &p, const double *vals, double *xbuf, int M, const int (&col)[SLOTS])
+{
+    if constexpr (SLOTS <= 16) {
+        if (p.d.taylor == kUnrolledTaylor) {
+            queue_loop<SLOTS, NPAIR, kUnrolledTaylor>(p, vals, xbuf, M, col);
+            return;
+        }
+    }
+    queue_loop<SLOTS, NPAIR, 0>(p, vals, xbuf, M, col);
+}
+
 template <int SLOTS>
-__global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
+// At most 128 VGPRs: the grid is sized for four wavefronts per SIMD.
+__global__ __launch_bounds__(kWave *kMaxWavesPerBlock)
+__attribute__((amdgpu_waves_per_eu(4))) void queue_kernel(
     QParams p)
 {
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
@@ -242,16 +253,16 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void queue_kernel(
         constexpr int kAll = (SLOTS - 2) / 2;
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
-                queue_loop<SLOTS, kAll>(p, lds, xbuf, M, col);
+                queue_loop_any<SLOTS, kAll>(p, lds, xbuf, M, col);
                 return;
             }
         }
         if (np >= 1) {
-            queue_loop<SLOTS, 1>(p, lds, xbuf, M, col);
+            queue_loop_any<SLOTS, 1>(p, lds, xbuf, M, col);
             return;
         }
     }
-    queue_loop<SLOTS, 0>(p, lds, xbuf, M, col);
+    queue_loop_any<SLOTS, 0>(p, lds, xbuf, M, col);
 }
 
 template <int SLOTS>
