@@ -131,7 +131,10 @@ def wide_adjoint_ell(ell, mats, dmats):
             if (r ^ 1) < n:
                 dval_e[:, 1, r] = dmats[:, r, r ^ 1]
             for k, c in enumerate(rows[r]):
-                dval_e[:, 2 + k, r] = dmats[:, r, c]
+                # a slot naming the row itself pads the matrices' pattern
+                # (its diagonal is slot 0)
+                if c != r:
+                    dval_e[:, 2 + k, r] = dmats[:, r, c]
     union_t = np.any(mats != 0, axis=0).T
     rows_t = [[int(c) for c in np.nonzero(union_t[r])[0]
                if int(c) not in (r, r ^ 1)] for r in range(n)]
