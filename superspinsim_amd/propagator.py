@@ -765,24 +765,23 @@ def choose_layout(pattern, layout="auto", pair_units=False):
     ``"plain"`` keeps the order.  ``"quad"`` and ``"pairs"`` return their
     searched order (``quad_order``, ``pair_fetch_order``; with
     ``pair_units`` the rows (2k, 2k + 1) stay together, and the pair layout
-    keeps the order).  ``"auto"`` takes the pair layout up to 32 rows and
-    the quad layout above, if it needs fewer LDS instructions per product
-    than the plain one, else the plain one.  Measured on the 7-level NV
-    sweep (4096 instances, same process): n = 49 plain 518, quad 474, pairs
-    490 ms; n = 19 plain 496, quad 460, pairs 445 ms.  With every lane
-    active the pair layout's scattered 16-byte reads meet more bank
-    conflicts; with half the lanes idle its fewer instructions win.
+    keeps the order).  ``"auto"`` takes the quad layout if it needs fewer
+    LDS instructions per product than the plain one, else the plain one.
+    Measured on the 7-level NV sweep (4096 instances, same process, unrolled
+    degree-11 chain, bank-aware slots): n = 49 plain 476, quad 430, pairs
+    477 ms; n = 19 plain 436, quad 409, pairs 412 ms.  The pair layout has
+    the fewest instructions, but its scattered 16-byte reads meet more bank
+    conflicts when every lane is active.
     """
     n = pattern.shape[0]
     ident = np.arange(n, dtype=np.int64)
     if layout == "plain" or n < 2:
         return ident, "plain"
     if layout == "auto":
-        layout = "pairs" if n <= 32 else "quad"
-        order, _ = choose_layout(pattern, layout, pair_units)
-        better = lds_instructions(pattern[order][:, order], layout) < \
+        order, _ = choose_layout(pattern, "quad", pair_units)
+        better = lds_instructions(pattern[order][:, order], "quad") < \
             lds_instructions(pattern, "plain")
-        return (order, layout) if better else (ident, "plain")
+        return (order, "quad") if better else (ident, "plain")
     if layout == "quad":
         return quad_order(pattern, pair_units=pair_units), "quad"
     if layout == "pairs":

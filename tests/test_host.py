@@ -245,10 +245,10 @@ def test_bank_aware_slot_order(layout, n):
 
 
 @pytest.mark.parametrize("use_incoherent,expect,instr",
-                         [(False, "quad", 5), (True, "pairs", 4)])
+                         [(False, "quad", 5), (True, "quad", 5)])
 def test_auto_layout_on_nv7(use_incoherent, expect, instr):
-    """The 7-level sweep: quads at n = 49, pairs at n = 19, each below the
-    plain layout's 6 LDS instructions per product."""
+    """The 7-level sweep (n = 49 and 19): quads, below the plain layout's 6
+    LDS instructions per product."""
     mats = _odmr7_pattern(use_incoherent)
     pat = np.any(mats != 0, axis=0)
     perm = pair_permutation(pat)
