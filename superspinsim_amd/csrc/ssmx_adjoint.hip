@@ -36,6 +36,7 @@
 
 namespace {
 
+using ssmx::kUnrolledTaylor;
 using ssmx::kWave;
 using ssmx::node_time;
 using ssmx::read_lane;
@@ -468,12 +469,26 @@ adjoint_kernel(AdjParams p)
                     }
                     W[t] = (v0 + v1) * (1.0 / t);
                 }
-                for (int t = MA; t <= m_fwd; ++t) {
+                // the rest of the recomputation's chain: for the common
+                // degree with constant 1 / t (a v_readlane result would
+                // stall each product), else the loop
+                if (m_fwd == kUnrolledTaylor) {
+#pragma unroll
+                    for (int t = MA; t <= kUnrolledTaylor; ++t) {
 #pragma clang fp contract(off)
-                    term = ell_product<SF>(aR, cf, term, x0, lane,
-                                           use_partner) *
-                           read_lane(inv_lane, t);
-                    acc += term;
+                        term = ell_product<SF>(aR, cf, term, x0, lane,
+                                               use_partner) *
+                               (1.0 / t);
+                        acc += term;
+                    }
+                } else {
+                    for (int t = MA; t <= m_fwd; ++t) {
+#pragma clang fp contract(off)
+                        term = ell_product<SF>(aR, cf, term, x0, lane,
+                                               use_partner) *
+                               read_lane(inv_lane, t);
+                        acc += term;
+                    }
                 }
                 if (r_on) {
                     x = acc;
