@@ -386,15 +386,23 @@ class GradientProblem:
         self._keep = (work, d_obs, seed)
         result = dict(theta=g_theta[:, :n_theta], records=g_seg)
         n_amp = state["amplitudes"].size
-        idx = state["tag_index"]
-        sel = np.nonzero(idx >= 0)[0]
-        g_amp = torch.zeros(n_amp, dtype=torch.float64, device="cuda")
-        if sel.size:
-            d_sel = torch.from_numpy(sel).to("cuda")
-            contrib = g_seg[d_sel] * torch.from_numpy(
-                state["tag_scale"][sel]).to("cuda")
-            g_amp.index_add_(0, torch.from_numpy(idx[sel]).to("cuda"),
-                             contrib)
+        if "amp_sum" not in state:
+            # records grouped by amplitude (stable: ascending record order),
+            # so that each amplitude's sum has a fixed order
+            idx = state["tag_index"]
+            sel = np.nonzero(idx >= 0)[0]
+            sel = sel[np.argsort(idx[sel], kind="stable")]
+            state["amp_sum"] = (
+                torch.from_numpy(sel).to("cuda"),
+                torch.from_numpy(state["tag_scale"][sel]).to("cuda"),
+                torch.from_numpy(np.bincount(idx[sel], minlength=n_amp))
+                .to("cuda"))
+        d_sel, d_scale, d_len = state["amp_sum"]
+        if d_sel.numel():
+            g_amp = torch.segment_reduce(g_seg[d_sel] * d_scale, "sum",
+                                         lengths=d_len)
+        else:
+            g_amp = torch.zeros(n_amp, dtype=torch.float64, device="cuda")
         result["amplitudes"] = g_amp
         if lambda_out:
             result["lambda"] = lam[:, prop.d_inv_perm]
