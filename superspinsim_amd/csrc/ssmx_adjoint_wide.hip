@@ -425,8 +425,10 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
             for (int j = J - 1; j >= 0; --j) {
                 ws_store(ws + ((size_t)(k - k0) * J + j) * NP + row, x);
                 e.base = j * M;
-                x = wide_exp_dispatch<SF>(ws_f, x, e, row, factor, inv_lane,
-                                          own, gpf, parity);
+                // the loop form (same bits as the propagator's unrolled
+                // chain): this kernel is at its register limit
+                x = wide_exp_dispatch<SF, false>(ws_f, x, e, row, factor,
+                                                 inv_lane, own, gpf, parity);
             }
         }
         if (parity) {

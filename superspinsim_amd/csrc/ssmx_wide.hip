@@ -243,8 +243,8 @@ __global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
             const double x_prev = x;
             for (int j = J - 1; j >= 0; --j) {
                 e.base = (b * J + j) * M;
-                x = wide_exp_dispatch<SLOTS>(ws, x, e, row, factor, inv_lane,
-                                             own, gp, parity);
+                x = wide_exp_dispatch<SLOTS, true>(ws, x, e, row, factor,
+                                                   inv_lane, own, gp, parity);
             }
             if (rotating) {
                 const double xp = swap_pair(x);

@@ -73,10 +73,34 @@ __device__ __forceinline__ double wide_product(
     return y0 + y1;
 }
 
+// Taylor terms t = T..M of one exponential, unrolled: term <- (A term) / t,
+// acc += term, product t on buffer (P0 + t - 1) & 1, 1 / t constants (the
+// same doubles as kInvW, so the same bits as the loop in wide_exp).
+template <int SLOTS, int WS, int M, int P0, int T = 1>
+__device__ __forceinline__ void wide_chain(const double (&a)[WS],
+                                           double &term, double &acc,
+                                           double *own,
+                                           const double *const (&gp)[SLOTS],
+                                           bool partner)
+{
+    if constexpr (T <= M) {
+        {
+#pragma clang fp contract(off)
+            term = wide_product<SLOTS, WS, (P0 + T - 1) & 1>(a, term, own,
+                                                            gp, partner) *
+                   (1.0 / T);
+            acc += term;
+        }
+        wide_chain<SLOTS, WS, M, P0, T + 1>(a, term, acc, own, gp, partner);
+    }
+}
+
 // x <- exp(A_j) x for this thread's row: assemble the row of A_j from the
 // staged values (slots < WS), then s substeps of the degree-m Taylor
-// action.  `parity` is the buffer the next product writes.
-template <int SLOTS, int WS>
+// action.  `parity` is the buffer the next product writes.  With UNROLL,
+// degree kUnrolledTaylor runs unrolled (wide_chain): a v_readlane of 1 / t
+// would stall every product.  Both forms give the same bits.
+template <int SLOTS, int WS, bool UNROLL>
 __device__ __forceinline__ double wide_exp(double x, const ExpArgs &e,
                                            int row, double factor,
                                            double inv_lane, double *own,
@@ -118,6 +142,18 @@ __device__ __forceinline__ double wide_exp(double x, const ExpArgs &e,
     for (int sub = 0; sub < e.substeps; ++sub) {
         double term = x;
         double acc = x;
+        if (UNROLL && e.taylor == ssmx::kUnrolledTaylor) {
+            constexpr int kM = ssmx::kUnrolledTaylor;
+            if (parity)
+                wide_chain<SLOTS, WS, kM, 1>(a, term, acc, own, gp,
+                                             e.partner);
+            else
+                wide_chain<SLOTS, WS, kM, 0>(a, term, acc, own, gp,
+                                             e.partner);
+            parity ^= kM & 1;
+            x = acc;
+            continue;
+        }
         // products alternate buffers; the loop body is one pair (buffer 0,
         // then 1), so both buffer offsets are immediates
         int t = 1;
@@ -148,21 +184,21 @@ __device__ __forceinline__ double wide_exp(double x, const ExpArgs &e,
 }
 
 // wide_exp for the smallest even slot class WS >= ws (wave-uniform).
-template <int SLOTS, int WS = 2>
+template <int SLOTS, bool UNROLL, int WS = 2>
 __device__ __forceinline__ double wide_exp_dispatch(
     int ws, double x, const ExpArgs &e, int row, double factor,
     double inv_lane, double *own, const double *const (&gp)[SLOTS],
     int &parity)
 {
     if constexpr (WS >= SLOTS) {
-        return wide_exp<SLOTS, SLOTS>(x, e, row, factor, inv_lane, own, gp,
-                                      parity);
+        return wide_exp<SLOTS, SLOTS, UNROLL>(x, e, row, factor, inv_lane,
+                                              own, gp, parity);
     } else {
         if (ws <= WS)
-            return wide_exp<SLOTS, WS>(x, e, row, factor, inv_lane, own, gp,
-                                       parity);
-        return wide_exp_dispatch<SLOTS, WS + 2>(ws, x, e, row, factor,
-                                                inv_lane, own, gp, parity);
+            return wide_exp<SLOTS, WS, UNROLL>(x, e, row, factor, inv_lane,
+                                               own, gp, parity);
+        return wide_exp_dispatch<SLOTS, UNROLL, WS + 2>(
+            ws, x, e, row, factor, inv_lane, own, gp, parity);
     }
 }
 
