@@ -371,48 +371,35 @@ def hyperfine_leg(args):
 
 
 def secondary_leg(name):
-    """Steps/s of another sweep config (one full launch, inputs resident)."""
+    """Steps/s of another sweep config (one full launch, inputs resident;
+    small models run packed, several instances per wavefront)."""
     import numpy as np
     import torch
-    from superspinsim_amd.propagator import device_tensor
-    from superspinsim_amd.sweep import pack_waveforms
     wl = workload(name, None)
     sim = wl.simulator()
     waves = wl.waveforms()
     n_inst = len(waves)
-    sim.run(waves, wl.rho0, 0.0, wl.time_step * (1 + 1e-12), wl.time_step,
-            obs=wl.obs, read="obs")
+    run = sim.prepare(waves, wl.rho0, wl.time_start, wl.time_end,
+                      wl.time_step, obs=wl.obs, read="obs")
     plan = dict(sim.last_plan)
-    prop, program, m, sub = sim.plan(waves, wl.time_step)
-    d_waves, d_segs = pack_waveforms(sim.kernel_waves(waves, program),
-                                     len(program))
-    d_rho = device_tensor(np.repeat(wl.rho0[None], n_inst, axis=0))
-    out = torch.empty((n_inst, wl.n_out, 1), dtype=torch.float64,
-                      device="cuda")
-
-    def launch():
-        prop.run(n_inst=n_inst, fine=wl.fine, n_out=wl.n_out,
-                 h=wl.time_step / wl.fine, t0=0.0, dt=wl.time_step,
-                 rho0=d_rho, waves=d_waves, segments=d_segs, taylor=m,
-                 substeps=sub, read="obs", obs=wl.obs, out=out)
-
-    launch()
+    run.launch()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     t0 = time.perf_counter()
     ev[0].record(stream)
-    launch()
+    run.launch()
     ev[1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    out = run.result()
     steps = n_inst * wl.steps_per_instance()
-    plan["steps"] = wl.steps_per_instance()
     flops = sim.step_flops(plan) * steps / (ev[0].elapsed_time(ev[1]) / 1e3)
     return {"workload": wl.name, "instances": n_inst,
             "superoperator_n": wl.n,
             "steps_per_instance": wl.steps_per_instance(),
-            "taylor_degree": m, "ell_slots": plan["slots"],
+            "taylor_degree": plan["taylor"], "ell_slots": plan["slots"],
+            "instances_per_wavefront": plan["packed"],
             "s_per_sweep": wall, "steps_per_s": steps / wall,
             "achieved_tflops": flops / 1e12,
             "finite": bool(np.all(np.isfinite(out.cpu().numpy())))}

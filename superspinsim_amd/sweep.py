@@ -154,15 +154,20 @@ class BatchSimulator:
     order of the coefficient rows, the CF scheme, fine divisions and the
     optional rotating frame (eigen-data from ``make_rotating``; generators
     then exclude L0).  ``layout`` is the propagators' ELL layout
-    (``Propagator``).
+    (``Propagator``).  ``pack`` lets ``run`` put several instances of a
+    small model (n <= 32, lab frame) into one block-diagonal system per
+    wavefront (``pack_factor``); an integer asks for that many per system
+    (within the kernel's limits) whatever the instance count.
     """
 
     def __init__(self, generators, number_of_exponentials=2,
                  number_of_fine_divisions=1, use_rotating=False,
                  vectors_real=None, inv_vectors_real=None, doubles=None,
-                 singles=None, tolerance=1e-16, layout="auto"):
+                 singles=None, tolerance=1e-16, layout="auto",
+                 pack=True):
         self.gens = np.asarray(generators, dtype=_f64)
         self.layout = layout
+        self.pack = pack if isinstance(pack, bool) else int(pack)
         self.G, self.n = self.gens.shape[0], self.gens.shape[1]
         self.nodes, self.weights, self.label = quadratures.scheme(
             number_of_exponentials)
@@ -212,6 +217,68 @@ class BatchSimulator:
                 mats, self.weights, self.nodes, rotating=self.rotating,
                 eig=eig, layout=self.layout)
         return self._prop_cache[key]
+
+    def _packed_propagator(self, active, const, time_step, k):
+        """Propagator of k instances in one block-diagonal system.
+
+        Block j (rows j n .. j n + n - 1) is instance j of the pack.  The
+        folded constant matrix C becomes diag(C, .., C) with coefficient 1;
+        each varying generator G becomes k matrices, G in block j only,
+        whose coefficient is member j's.  A row of block j therefore sums
+        exactly the terms of the single-instance row (the other blocks'
+        matrices add exact zeros).
+        """
+        key = ("packed", k, tuple(active), tuple(sorted(const.items())),
+               float(time_step))
+        if key not in self._prop_cache:
+            n = self.n
+            varying = [g for g in active if g not in const]
+            parts = []
+            if const:
+                c = sum(v * self.gens[g] for g, v in const.items())
+                parts.append(np.kron(np.eye(k), c)[None])
+            for g in varying:
+                for j in range(k):
+                    m = np.zeros((k * n, k * n))
+                    m[j * n:(j + 1) * n, j * n:(j + 1) * n] = self.gens[g]
+                    parts.append(m[None])
+            self._prop_cache[key] = Propagator(
+                np.concatenate(parts, axis=0), self.weights, self.nodes,
+                layout=self.layout)
+        return self._prop_cache[key]
+
+    def pack_factor(self, n_inst, program, read="obs", n_obs=1):
+        """Instances per block-diagonal system for one ``run``.
+
+        A wavefront computes 64 rows whatever n is, so with n <= 32 several
+        instances share one: k <= 64 / n, within the kernel's generator
+        count (the constant block plus k copies of each varying generator)
+        and readout rows (k n_obs <= 2; a full-state readout has no such
+        limit).  k also keeps at least one system per SIMD (four per CU),
+        below which packing would only idle SIMDs.  1 in the rotating
+        frame or when ``pack`` is off.
+        """
+        if not self.pack or self.rotating or self.n > 32 or n_inst < 2:
+            return 1
+        n_const = 1 if None in program else 0
+        n_var = len(program) - n_const
+        J = self.weights.shape[0]
+        k = _lib.MAX_N // self.n
+        if n_var:
+            # generators per system, and one coefficient batch of 64 lanes
+            # holding J of them per step
+            k = min(k, (_lib.MAX_GEN - n_const) // n_var,
+                    (_lib.MAX_N // J - n_const) // n_var)
+        if read != "full":
+            k = min(k, _lib.MAX_OBS // max(n_obs, 1))
+        if self.pack is True:
+            torch = __import__("torch")
+            simds = 4 * torch.cuda.get_device_properties(
+                0).multi_processor_count
+            k = min(k, max(1, n_inst // simds))
+        else:
+            k = min(k, int(self.pack))          # a fixed request
+        return max(1, k)
 
     def program(self, waves):
         """(active, const, program) of a batch of waveforms.
@@ -268,6 +335,73 @@ class BatchSimulator:
             out.append(nw)
         return out
 
+    def prepare(self, waves, rho0, time_start, time_end, time_step,
+                obs=None, read="obs", taylor=None, substeps=None):
+        """Plan, pack and stage one ``run`` on the device; returns a
+        ``PreparedRun`` whose ``launch`` only enqueues the kernel."""
+        N = int((time_end - time_start) / time_step)
+        prop0, program, m, s = self.plan(waves, time_step)
+        prop = prop0
+        if taylor is not None:
+            m = taylor
+        if substeps is not None:
+            s = substeps
+        n_inst = len(waves)
+        rho0 = np.atleast_2d(np.asarray(rho0, dtype=_f64))
+        if rho0.shape[0] == 1 and n_inst > 1:
+            rho0 = np.repeat(rho0, n_inst, axis=0)
+        x0 = rho0 @ self.Vinv.T if self.rotating else rho0
+        obs_eff = None
+        if obs is not None:
+            obs = np.atleast_2d(np.asarray(obs, dtype=_f64))
+            obs_eff = obs @ self.V if self.rotating else obs
+        n_obs = obs_eff.shape[0] if obs_eff is not None else 0
+        k = self.pack_factor(n_inst, program, read, n_obs)
+        kwaves = self.kernel_waves(waves, program)
+        if k > 1:
+            # systems of k consecutive instances; the last is filled up with
+            # copies of the last instance, whose results are dropped
+            n_sys = -(-n_inst // k)
+            fill = [min(i, n_inst - 1) for i in range(n_sys * k)]
+            active, const, _ = self.program(waves)
+            prop = self._packed_propagator(active, const, time_step, k)
+            n_const = 1 if None in program else 0
+            kwaves = [self._pack_waves([kwaves[fill[q * k + j]]
+                                        for j in range(k)], n_const)
+                      for q in range(n_sys)]
+            x0 = x0[fill].reshape(n_sys, k * self.n)
+            if obs_eff is not None:
+                blk = np.zeros((k * n_obs, k * self.n))
+                for j in range(k):
+                    blk[j * n_obs:(j + 1) * n_obs,
+                        j * self.n:(j + 1) * self.n] = obs_eff
+                obs_eff = blk
+        d_waves, d_segs = pack_waveforms(kwaves, kwaves[0].n_gen)
+        run = PreparedRun(self, prop, k, n_inst, read, N, m, s, time_start,
+                          time_step, device_tensor(x0), d_waves, d_segs,
+                          obs_eff)
+        self.last_plan = dict(taylor=m, substeps=s, slots=prop.ell.slots,
+                              nnz=prop0.ell.nnz, active=program, n_out=N,
+                              steps=N * self.F, n=self.n, packed=k)
+        return run
+
+    @staticmethod
+    def _pack_waves(members, n_const):
+        """Kernel waveform of one block-diagonal system: the constant slot,
+        then for each varying generator one slot per member."""
+        k = len(members)
+        n_var = members[0].n_gen - n_const
+        w = Waveform(n_const + k * n_var)
+        if n_const:
+            w.base[0] = 1.0
+        for v in range(n_var):
+            for j, mw in enumerate(members):
+                g, slot = n_const + v, n_const + v * k + j
+                w.base[slot] = mw.base[g]
+                w.segments[slot] = list(mw.segments[g])
+                w.params[slot] = list(mw.params[g])
+        return w
+
     def run(self, waves, rho0, time_start, time_end, time_step, obs=None,
             read="obs", taylor=None, substeps=None):
         """Propagate every instance; returns a torch tensor on the device.
@@ -277,35 +411,12 @@ class BatchSimulator:
         ``read`` is "obs" (every output), "integral" (trapezoid over all
         propagator steps) or "full" (state vectors, standard basis).
         """
-        torch = __import__("torch")
-        N = int((time_end - time_start) / time_step)
-        prop, program, m, s = self.plan(waves, time_step)
-        if taylor is not None:
-            m = taylor
-        if substeps is not None:
-            s = substeps
-        d_waves, d_segs = pack_waveforms(self.kernel_waves(waves, program),
-                                         len(program))
-        rho0 = np.atleast_2d(np.asarray(rho0, dtype=_f64))
-        if rho0.shape[0] == 1 and len(waves) > 1:
-            rho0 = np.repeat(rho0, len(waves), axis=0)
-        x0 = rho0 @ self.Vinv.T if self.rotating else rho0
-        obs_eff = None
-        if obs is not None:
-            obs = np.atleast_2d(np.asarray(obs, dtype=_f64))
-            obs_eff = obs @ self.V if self.rotating else obs
-        out = prop.run(n_inst=len(waves), fine=self.F, n_out=N,
-                       h=time_step / self.F, t0=time_start, dt=time_step,
-                       rho0=device_tensor(x0), waves=d_waves,
-                       segments=d_segs, taylor=m, substeps=s, read=read,
-                       obs=obs_eff)
-        if read == "full" and self.rotating:
-            out = out @ torch.from_numpy(self.V.T.copy()).to(out.device)
-        self.last_plan = dict(taylor=m, substeps=s, slots=prop.ell.slots,
-                              nnz=prop.ell.nnz, active=program, n_out=N,
-                              steps=N * self.F, n=self.n)
-        self._keep = (d_waves, d_segs)
-        return out
+        run = self.prepare(waves, rho0, time_start, time_end, time_step,
+                           obs=obs, read=read, taylor=taylor,
+                           substeps=substeps)
+        run.launch()
+        self._keep = run
+        return run.result()
 
     def step_flops(self, plan=None):
         """Algorithmic FP64 FLOPs of one propagator step of one instance.
@@ -321,3 +432,49 @@ class BatchSimulator:
         per_exp = 2 * K * nnz + n + p["substeps"] * p["taylor"] * (
             2 * nnz + 2 * n)
         return J * per_exp + (4 * n if self.rotating else 0)
+
+
+class PreparedRun:
+    """Device inputs of one ``BatchSimulator.run`` (see ``prepare``):
+    ``launch`` enqueues the kernel on the current stream, ``result`` maps
+    the output of the last launch back to one row per instance (standard
+    basis for full states)."""
+
+    def __init__(self, sim, prop, k, n_inst, read, N, taylor, substeps, t0,
+                 dt, d_x0, d_waves, d_segs, obs):
+        torch = __import__("torch")
+        self.sim, self.prop, self.k, self.n_inst = sim, prop, k, n_inst
+        self.read, self.N = read, N
+        self.args = dict(n_inst=d_x0.shape[0], fine=sim.F, n_out=N,
+                         h=dt / sim.F, t0=t0, dt=dt, rho0=d_x0,
+                         waves=d_waves, segments=d_segs, taylor=taylor,
+                         substeps=substeps, read=read, obs=obs)
+        n_obs = obs.shape[0] if obs is not None else 0
+        shape = {"full": (d_x0.shape[0], N, prop.n),
+                 "obs": (d_x0.shape[0], N, n_obs),
+                 "integral": (d_x0.shape[0], n_obs)}[read]
+        self.out = torch.empty(shape, dtype=torch.float64, device="cuda")
+        self.last = None
+
+    def launch(self):
+        self.last = self.prop.run(out=self.out, **self.args)
+        return self.last
+
+    def result(self):
+        out, k, n = self.last, self.k, self.sim.n
+        if k > 1:
+            S = out.shape[0]
+            if self.read == "full":
+                out = out.reshape(S, self.N, k, n).permute(0, 2, 1, 3)
+                out = out.reshape(S * k, self.N, n)
+            elif self.read == "obs":
+                n_obs = out.shape[2] // k
+                out = out.reshape(S, self.N, k, n_obs).permute(0, 2, 1, 3)
+                out = out.reshape(S * k, self.N, n_obs)
+            else:
+                out = out.reshape(S * k, -1)
+            out = out[:self.n_inst]
+        if self.read == "full" and self.sim.rotating:
+            torch = __import__("torch")
+            out = out @ torch.from_numpy(self.sim.V.T.copy()).to(out.device)
+        return out
