@@ -36,11 +36,13 @@
  *    used-slot counts let each wavefront skip slots none of its rows use.
  *    For n <= 64 with partner = 1, a slot pair (2 + 2k, 3 + 2k) whose two
  *    columns are 2p and 2p + 1 (an aligned pair) in every row of the
- *    64-row table is read with one 16-byte LDS read instead of two
- *    gathers.  The kernels detect such leading slot pairs from ell_col
- *    (k = 0 only, or every slot pair); any column order gives the same
- *    result, and the layout only changes speed.  Rows >= n take part in
- *    the detection, so their slots must follow the layout too (values 0).
+ *    64-row table is read whole instead of by two gathers: when only the
+ *    first slot pair holds one (it is then the other pair of the row's
+ *    quad, (r & ~1) ^ 2 and its odd neighbour) by DPP, else by one 16-byte
+ *    LDS read each.  The kernels detect such leading slot pairs from
+ *    ell_col; any column order gives the same result, and the layout only
+ *    changes speed.  Rows >= n take part in the detection, so their slots
+ *    must follow the layout too (values 0).
  *  - One propagator step of size h applies exp(A_{J-1}) first and exp(A_0)
  *    last, with A_j = h sum_g (sum_s w[j,s] c[s,g]) G_g, or in the rotating
  *    frame A_j = sum_s sum_g (G~[s,g] c[s,g]) (h w[j,s]) followed by the

@@ -94,8 +94,9 @@ __device__ __forceinline__ int pad_column(int s, int lane)
 
 // Number of leading slot pairs (2 + 2k, 3 + 2k) that hold the two columns
 // 2p, 2p + 1 of an aligned pair in every lane's row, when slot 1 is the
-// partner r ^ 1 (else 0); wave-uniform.  Such slot pairs are read with one
-// 16-byte LDS read each.
+// partner r ^ 1 (else 0); wave-uniform.  Such slot pairs are read whole:
+// by DPP when only the first holds one (the other pair of the row's quad),
+// else by one 16-byte LDS read each.
 template <int SLOTS>
 __device__ __forceinline__ int fetched_pairs(const int (&col)[SLOTS],
                                              int lane, bool partner)
@@ -124,8 +125,9 @@ __device__ __forceinline__ int fetched_pairs(const int (&col)[SLOTS],
 // component r of v: one Taylor product.  Slot 0 is the lane's own
 // component.  Slot 1 is lane r ^ 1 by DPP when `partner` is set (always
 // when NPAIR > 0).  The first NPAIR slot pairs (2 + 2k, 3 + 2k) hold aligned
-// column pairs (fetched_pairs) and come from one 16-byte LDS read each; the
-// other slots are gathered one by one.  The LDS reads see `xbuf` (one
+// column pairs (fetched_pairs): with NPAIR = 1 the other pair of the quad,
+// read by DPP; otherwise each from one 16-byte LDS read.  The other slots
+// are gathered one by one.  The LDS reads see `xbuf` (one
 // double per lane of the wavefront, 16-byte aligned) after every lane has
 // written its component; LDS instructions of a wavefront execute in order,
 // so a wavefront-scope fence is the only barrier.  Even and odd slots are
@@ -146,11 +148,20 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
     if (SLOTS > 1)
         g[1] = (NPAIR > 0 || partner) ? swap_pair(v) : xbuf[col[1]];
     const double2 *x2 = reinterpret_cast<const double2 *>(xbuf);
+    if constexpr (NPAIR == 1) {
+        // the quad layout: the other pair of the quad is lanes
+        // (r & ~1) ^ 2 and its odd neighbour, in reach of DPP.  With the
+        // unrolled Taylor chain the four moves ran 1 % faster than the
+        // 16-byte read (with the Taylor loop, 6 % slower).
+        g[2] = quad_perm<0x0A>(v);  // [2, 2, 0, 0]
+        g[3] = quad_perm<0x5F>(v);  // [3, 3, 1, 1]
+    } else {
 #pragma unroll
-    for (int k = 0; k < NPAIR; ++k) {
-        const double2 o = x2[col[2 + 2 * k] >> 1];
-        g[2 + 2 * k] = o.x;
-        g[3 + 2 * k] = o.y;
+        for (int k = 0; k < NPAIR; ++k) {
+            const double2 o = x2[col[2 + 2 * k] >> 1];
+            g[2 + 2 * k] = o.x;
+            g[3 + 2 * k] = o.y;
+        }
     }
 #pragma unroll
     for (int sl = 2 + 2 * NPAIR; sl < SLOTS; ++sl)

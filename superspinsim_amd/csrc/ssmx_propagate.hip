@@ -26,11 +26,12 @@
 //    (LDS instructions of one wavefront execute in order, so no barrier is
 //    needed; half as many LDS instructions as two 32-bit ds_bpermute per
 //    double);
-//  - in the pair-fetch layout (slots 2 + 2k, 3 + 2k of every row are the
-//    two columns of an aligned pair 2p, 2p + 1, chosen by the host's basis
-//    order) each slot pair comes from one 16-byte LDS read instead
-//    (ssmx::ell_apply); the kernel recognises that layout from the column
-//    table.
+//  - in the quad layout (slots 2, 3 of every row are the other pair of its
+//    quad) those two come by DPP as well; in the pair-fetch layout (slots
+//    2 + 2k, 3 + 2k of every row are the two columns of an aligned pair
+//    2p, 2p + 1) each slot pair comes from one 16-byte LDS read
+//    (ssmx::ell_apply).  The host picks the basis order; the kernel
+//    recognises the layout from the column table.
 // Coefficients are evaluated lane-parallel in batches: lane e of a batch
 // computes the scalar factor of (step, exponential j, matrix m); the factors
 // are broadcast to the wavefront with readlane when the step runs.

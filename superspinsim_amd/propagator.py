@@ -132,8 +132,8 @@ def quad_columns(r):
 def quad_gathers(pattern):
     """Largest number of columns of a row outside its quad (rows 4q..4q+3).
 
-    These are the single LDS gathers per product left in the quad layout,
-    beside the write and the 16-byte read of the quad's other pair.
+    These are the LDS gathers per product left in the quad layout, beside
+    the write; the quad's other lanes come by DPP.
     """
     need = _off_diagonal(pattern)
     n = need.shape[0]
@@ -145,9 +145,8 @@ def quad_gathers(pattern):
 def quad_order(pattern, pair_units=False, restarts=8, seed=0):
     """Basis order that puts coupled rows into common quads (4q .. 4q + 3).
 
-    Within a quad every row reads its partner lane by DPP and the other
-    pair of the quad by one 16-byte LDS read, so each column outside its
-    quad costs one more LDS instruction.  The order minimises, lexicographically, the largest count of such
+    Within a quad every row reads the three other lanes by DPP, so only
+    the columns outside its quad cost an LDS gather.  The order minimises, lexicographically, the largest count of such
     columns over the rows (the wavefront executes that many gathers per
     product), the number of rows at that count, and their sum, by steepest
     descent over swaps between quads, from the current order and from a few
@@ -449,7 +448,8 @@ class EllStack:
       2p, 2p + 1 of the row's k-th fetched pair (``fetch_pairs``).
 
     In the last two every row of the padded table follows the layout, and
-    the kernels read each such slot pair with one 16-byte LDS read.  With
+    the kernels read such slot pairs whole (the quad's by DPP, the pairs
+    by one 16-byte LDS read each).  With
     ``bank_aware`` the fetched addresses are ordered over the slots so that
     lanes read by one LDS cycle meet fewer bank conflicts.  ``col`` is int32
     (slots, NP), ``val`` float64 (M, slots, NP) with NP = 64 ceil(n / 64);
@@ -755,7 +755,7 @@ def lds_instructions(pattern, layout):
     if layout == "plain":
         return 1 + EllStack(pattern[None].astype(_f64)).gather_slots
     if layout == "quad":
-        return 2 + quad_gathers(pattern)
+        return 1 + quad_gathers(pattern)
     return 1 + pair_fetches(pattern)
 
 

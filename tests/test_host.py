@@ -245,7 +245,7 @@ def test_bank_aware_slot_order(layout, n):
 
 
 @pytest.mark.parametrize("use_incoherent,expect,instr",
-                         [(False, "quad", 5), (True, "quad", 5)])
+                         [(False, "quad", 4), (True, "quad", 4)])
 def test_auto_layout_on_nv7(use_incoherent, expect, instr):
     """The 7-level sweep (n = 49 and 19): quads, below the plain layout's 6
     LDS instructions per product."""
