@@ -96,6 +96,8 @@ def parse():
     p.add_argument("--hyperfine-adjoint-instances", type=int, default=512)
     p.add_argument("--hyperfine-adjoint-time", type=float, default=5e-6,
                    help="simulated time of the gradient sweep (s)")
+    p.add_argument("--hyperfine-adjoint-incoherent", action="store_true",
+                   help="gradient leg on the compressed model (n = 171)")
     return p.parse_args()
 
 
@@ -406,13 +408,16 @@ def secondary_leg(name):
 
 
 def hyperfine_adjoint_leg(args):
-    """Forward + adjoint of the 21-level sweep (n = 441): gradients of the
-    integrated fluorescence over the 7-level and 14N parameters."""
+    """Forward + adjoint of the 21-level sweep (n = 441, or 171 with
+    --hyperfine-adjoint-incoherent): gradients of the integrated
+    fluorescence over the 7-level and 14N parameters."""
     import numpy as np
     import torch
     from superspinsim_amd import configs
-    wl = configs.odmr21_gradient(count=args.hyperfine_adjoint_instances,
-                                 time_end=args.hyperfine_adjoint_time)
+    wl = configs.odmr21_gradient(
+        count=args.hyperfine_adjoint_instances,
+        time_end=args.hyperfine_adjoint_time,
+        use_incoherent=args.hyperfine_adjoint_incoherent)
     pb = wl.problem
     t_setup = time.perf_counter()
     state = pb.setup(wl.theta, wl.amplitudes)
@@ -612,7 +617,10 @@ def main():
     if not args.no_hyperfine and args.config == "odmr7":
         result["hyperfine"] = hyperfine_leg(args)
     if not args.no_hyperfine_adjoint and args.config == "odmr7":
-        result["hyperfine_adjoint"] = hyperfine_adjoint_leg(args)
+        for inc, key in ((False, "hyperfine_adjoint"),
+                         (True, "hyperfine_adjoint_n171")):
+            args.hyperfine_adjoint_incoherent = inc
+            result[key] = hyperfine_adjoint_leg(args)
     if not args.no_cpu:
         outputs = args.cpu_outputs or (128 if wl.n > 20 else 999)
         result["cpu_baseline"] = cpu_baseline(wl, outputs)

@@ -122,12 +122,13 @@ struct RevArgs {
     int row;
     double factor;      // this lane's factor (lane j M + m)
     double fth;         // factor of the matrix the derivative tables follow
-    double inv_lane;    // 1 / lane
     double *own;        // the row's entry in exchange buffer 0
 };
 
 // Step I (1..MA) of the transposed chain on buffer (I - 1) & 1:
-// u <- (A^T u) / I, lambda += u, v_t += c_It u for t < MA - I.
+// u <- (A^T u) / I, lambda += u, v_t += c_It u for t < MA - I.  The
+// compile-time 1.0 / I is the same double as kInvW[I] and needs no
+// v_readlane (whose SGPR result stalls the next VALU instruction).
 template <int SF, int WT, int MA, int I>
 __device__ __forceinline__ void u_chain(double &u, double &acc,
                                         double (&v)[MA],
@@ -136,7 +137,7 @@ __device__ __forceinline__ void u_chain(double &u, double &acc,
                                         const double *const (&gpt)[SF])
 {
     u = wide_product<SF, WT, (I - 1) & 1>(at, u, r.own, gpt, true) *
-        read_lane(r.inv_lane, I);
+        (1.0 / I);
     acc += u;
 #pragma unroll
     for (int t = 0; t < MA - I; ++t)
@@ -178,7 +179,7 @@ __device__ __forceinline__ void w_chain(double &w, const double (&v)[MA],
             else
                 y0 = fma(a[sl], g[sl], y0);
         }
-        w = (y0 + y1) * read_lane(r.inv_lane, T + 1);
+        w = (y0 + y1) * (1.0 / (T + 1));
         w_chain<SE, WE, WA, MA, T + 1>(w, v, B, a, r, gpe);
     }
 }
@@ -453,7 +454,6 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
                 r.row = row;
                 r.factor = factor;
                 r.fth = tg >= 0 ? read_lane(factor, j * M + tg) : 0.0;
-                r.inv_lane = inv_lane;
                 r.own = own;
                 double v[MA];
                 reverse_u_dispatch<SF, MA>(ws_t, lam, v, r, gpt, src);
@@ -588,6 +588,8 @@ int ssmx_adjoint_wide(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
     };
     if (fits(8) && se <= 16)
         return pick_degree<8, 16>(p, stream);
+    if (fits(12) && se <= 18)
+        return pick_degree<12, 18>(p, stream);
     if (fits(12) && se <= 20)
         return pick_degree<12, 20>(p, stream);
     if (fits(16) && se <= 22)

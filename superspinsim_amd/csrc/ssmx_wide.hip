@@ -4,9 +4,11 @@
 //
 // W = ceil(n / 64) wavefronts advance one instance: thread r of the
 // instance owns row r of the superoperator and component r of the state.
-// A workgroup holds IPB = floor(16 / W) instances (2 at n = 441, 5 at
-// n = 171), which share one LDS copy of the generator values (every
-// instance uses the same matrices; only the coefficients differ).  The
+// A workgroup holds IPB <= floor(16 / W) instances, which share one LDS
+// copy of the generator values (every instance uses the same matrices;
+// only the coefficients differ).  The launch puts one instance in a
+// workgroup when two such workgroups fit on a CU (n = 171), else the count
+// that keeps the most instances resident (2 at n = 441).  The
 // Taylor action exp(A) x is evaluated as in ssmx_propagate.hip, with the
 // exchange of each term across the instance's wavefronts:
 //  - slot 0 (diagonal) needs no exchange, slot 1 (partner r ^ 1, same
@@ -289,30 +291,58 @@ int launch(const WParams &p, hipStream_t stream)
     const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
     const int W = (p.d.n + kWave - 1) / kWave;
     const int NP = W * kWave;
-    // instances per workgroup: up to 16 wavefronts, and the staged values
+    // instances per workgroup: at most 16 wavefronts, and the staged values
     // plus two exchange buffers per instance within 160 KiB of LDS
     const size_t staged = (size_t)M * SLOTP * NP * sizeof(double);
-    int ipb = kMaxBlockWaves / W;
-    if (ipb > p.d.n_inst)
-        ipb = p.d.n_inst;
-    while (ipb > 1 && staged + (size_t)ipb * (2 * kBuf + W) *
-                                   sizeof(double) > 160 * 1024)
-        --ipb;
-    const size_t lds =
-        staged + (size_t)ipb * (2 * kBuf + W) * sizeof(double);
-    if (lds > 160 * 1024)
+    auto lds_of = [&](int ipb) {
+        return staged + (size_t)ipb * (2 * kBuf + W) * sizeof(double);
+    };
+    int ipb_max = kMaxBlockWaves / W;
+    if (ipb_max > p.d.n_inst)
+        ipb_max = p.d.n_inst;
+    while (ipb_max > 1 && lds_of(ipb_max) > 160 * 1024)
+        --ipb_max;
+    if (lds_of(ipb_max) > 160 * 1024)
         return set_error(SSMX_EINVAL,
                          "n=%d: generator values need %zu bytes of LDS "
                          "(> 160 KiB)",
-                         p.d.n, lds);
-    if (lds > 64 * 1024) {
+                         p.d.n, lds_of(ipb_max));
+    const void *kern = reinterpret_cast<const void *>(&wide_kernel<SLOTS>);
+    if (lds_of(ipb_max) > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&wide_kernel<SLOTS>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+            (int)lds_of(ipb_max));
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
+    // One instance per workgroup when two or more such workgroups fit on a
+    // CU: fewer wavefronts meet at each barrier.  On the n = 171 model this
+    // ran 512-4096 points 1.2-1.8x faster than 5 per workgroup (and
+    // within 1 % of it at 1024).  Otherwise (n = 441: the staged values
+    // allow one workgroup per CU) the count that keeps the most instances
+    // resident per CU.
+    auto resident = [&](int c) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, kern, c * NP, lds_of(c)) != hipSuccess) {
+            (void)hipGetLastError();    // not a launch error: count 0
+            return 0;
+        }
+        return nb;
+    };
+    int ipb = 1;
+    if (resident(1) < 2) {
+        int most = -1;
+        for (int c = 1; c <= ipb_max; ++c) {
+            const int r = resident(c) * c;
+            if (r > most) {
+                most = r;
+                ipb = c;
+            }
+        }
+    }
+    const size_t lds = lds_of(ipb);
     const int blocks = (p.d.n_inst + ipb - 1) / ipb;
     hipLaunchKernelGGL((wide_kernel<SLOTS>), dim3(blocks),
                        dim3(ipb * NP), lds, stream, p, W, ipb);
