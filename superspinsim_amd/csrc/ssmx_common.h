@@ -179,12 +179,61 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
     return y0 + y1;
 }
 
+// 1.0 / T (correctly rounded, as the compiler folds it) built by two
+// scalar moves at the point of use: as plain constants the compiler hoists
+// all of them out of the step loop, where they spill and come back through
+// v_readlane, whose SGPR result stalls the next multiply.
+template <int T>
+__device__ __forceinline__ double scalar_inverse()
+{
+    constexpr uint64_t bits = __builtin_bit_cast(uint64_t, 1.0 / T);
+    uint32_t lo, hi;
+    asm volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"((uint32_t)bits));
+    asm volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"((uint32_t)(bits >> 32)));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Terms T..M of taylor_action's unrolled chain, stopping after term m
+// (template recursion, so every 1.0 / T is a compile-time constant).
+template <int SLOTS, int NPAIR, int M, int T = 1>
+__device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
+                                             const int (&col)[SLOTS],
+                                             double &term, double &acc,
+                                             double *xbuf, int lane,
+                                             bool partner, int m)
+{
+    if constexpr (T <= M) {
+        if (T > m)
+            return;
+        // 1 and 1/2 stay plain constants (the multiply by 1 folds away, 0.5
+        // is an inline operand)
+        double inv;
+        if constexpr (T <= 2)
+            inv = 1.0 / T;
+        else
+            inv = scalar_inverse<T>();
+        {
+#pragma clang fp contract(off)
+            term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
+                                           partner) *
+                   inv;
+            acc += term;
+        }
+        taylor_chain<SLOTS, NPAIR, M, T + 1>(a, col, term, acc, xbuf, lane,
+                                             partner, m);
+    }
+}
+
 // exp(A) x by its Taylor series of degree m: term <- (A term) * (1 / t),
 // acc += term for t = 1..m, each product and sum rounded on its own (no
 // contraction), so every kernel that repeats the chain gets the same bits.
-// M > 0: the degree is the compile-time M, the chain is unrolled and 1 / t
-// are constants; M = 0: the runtime degree m, 1 / t broadcast from lane t
-// of inv_lane (which holds 1 / lane).
+// M > 0: the degree is M, the chain is unrolled and the 1 / t are plain
+// constants.  M < 0: the chain is unrolled up to -M terms and stops after
+// the runtime degree m <= -M (one scalar branch per term), each 1 / t built
+// at its use (scalar_inverse).  M = 0: a loop over the runtime degree m,
+// 1 / t broadcast from lane t of inv_lane (which holds 1 / lane).  All
+// three give the same doubles in the same order; the unrolled forms avoid
+// the loop's v_readlane, whose SGPR result stalls the dependent chain.
 template <int SLOTS, int NPAIR, int M>
 __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
                                                 const int (&col)[SLOTS],
@@ -203,6 +252,15 @@ __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
                    (1.0 / t);
             acc += term;
         }
+    } else if constexpr (M < 0) {
+        // The degree through an opaque scalar move: the per-term exit tests
+        // then stay scalar compares inside the chain, where the compiler
+        // would otherwise hoist all -M of them out of the step loop as lane
+        // masks and spill them (v_readlane reloads in the chain).
+        int mm;
+        asm volatile("s_mov_b32 %0, %1" : "=s"(mm) : "s"(m));
+        taylor_chain<SLOTS, NPAIR, -M>(a, col, term, acc, xbuf, lane,
+                                       partner, mm);
     } else {
         for (int t = 1; t <= m; ++t) {
 #pragma clang fp contract(off)
@@ -215,9 +273,16 @@ __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
     return acc;
 }
 
-// The Taylor degree with an unrolled chain (the degree the 1e-16
-// tolerance gives at the 20 ps steps of the NV workloads).
+// The Taylor degree with a chain unrolled for exactly that degree (the
+// degree the 1e-16 tolerance gives at the 20 ps steps of the NV
+// workloads): taylor_action<.., kUnrolledTaylor>, the adjoint's tail and
+// the wide kernels.
 constexpr int kUnrolledTaylor = 11;
+
+// Other degrees 1..kUnrolledMax run the chain with per-term exits
+// (taylor_action<.., -kUnrolledMax>) in the 64-row forward and queue
+// kernels.
+constexpr int kUnrolledMax = 16;
 
 // Node time of global output interval i, fine division r, node c, in the
 // reference's order without contraction:

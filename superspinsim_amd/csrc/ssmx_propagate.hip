@@ -43,6 +43,7 @@
 namespace {
 
 using ssmx::ell_apply;
+using ssmx::kUnrolledMax;
 using ssmx::kUnrolledTaylor;
 using ssmx::kWave;
 using ssmx::node_time;
@@ -285,21 +286,28 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
         p.state_out[(size_t)inst * n + lane] = x;
 }
 
-// The Taylor chain unrolled for the common degree (SLOTS <= 16), else the
-// runtime loop.
-template <int SLOTS, int NPAIR>
+// CHAIN > 0: the kernel variant launched for Taylor degree CHAIN, with the
+// chain unrolled for exactly that degree.  CHAIN = 0: the chain with
+// per-term exits for degrees up to kUnrolledMax (SLOTS <= 16), else the
+// runtime loop.  Separate kernels keep each chain's register allocation
+// apart (both in one kernel ran every degree 1-2 % slower).
+template <int SLOTS, int NPAIR, int CHAIN>
 __device__ __forceinline__ void propagate_instance_any(const Params &p, const double *vals, double *xbuf, int M, int inst, const int (&col)[SLOTS])
 {
-    if constexpr (SLOTS <= 16) {
-        if (p.d.taylor == kUnrolledTaylor) {
-            propagate_instance<SLOTS, NPAIR, kUnrolledTaylor>(p, vals, xbuf, M, inst, col);
-            return;
+    if constexpr (CHAIN > 0) {
+        propagate_instance<SLOTS, NPAIR, CHAIN>(p, vals, xbuf, M, inst, col);
+    } else {
+        if constexpr (SLOTS <= 16) {
+            if (p.d.taylor >= 1 && p.d.taylor <= kUnrolledMax) {
+                propagate_instance<SLOTS, NPAIR, -kUnrolledMax>(p, vals, xbuf, M, inst, col);
+                return;
+            }
         }
+        propagate_instance<SLOTS, NPAIR, 0>(p, vals, xbuf, M, inst, col);
     }
-    propagate_instance<SLOTS, NPAIR, 0>(p, vals, xbuf, M, inst, col);
 }
 
-template <int SLOTS>
+template <int SLOTS, int CHAIN>
 // At most 128 VGPRs: four wavefronts per SIMD (4096 instances fill the
 // device once).
 __global__ __launch_bounds__(kWave *kWavesPerBlock)
@@ -355,20 +363,20 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         constexpr int kAll = (SLOTS - 2) / 2;
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
-                propagate_instance_any<SLOTS, kAll>(p, lds, xbuf, M, inst, col);
+                propagate_instance_any<SLOTS, kAll, CHAIN>(p, lds, xbuf, M, inst, col);
                 return;
             }
         }
         if (np >= 1) {
-            propagate_instance_any<SLOTS, 1>(p, lds, xbuf, M, inst, col);
+            propagate_instance_any<SLOTS, 1, CHAIN>(p, lds, xbuf, M, inst, col);
             return;
         }
     }
-    propagate_instance_any<SLOTS, 0>(p, lds, xbuf, M, inst, col);
+    propagate_instance_any<SLOTS, 0, CHAIN>(p, lds, xbuf, M, inst, col);
 }
 
-template <int SLOTS>
-int launch(const Params &p, hipStream_t stream)
+template <int SLOTS, int CHAIN>
+int launch_chain(const Params &p, hipStream_t stream)
 {
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
     const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
@@ -381,20 +389,31 @@ int launch(const Params &p, hipStream_t stream)
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&propagate_kernel<SLOTS>),
+            reinterpret_cast<const void *>(&propagate_kernel<SLOTS, CHAIN>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
     const int blocks = (p.d.n_inst + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((propagate_kernel<SLOTS>), dim3(blocks),
+    hipLaunchKernelGGL((propagate_kernel<SLOTS, CHAIN>), dim3(blocks),
                        dim3(kWave * kWavesPerBlock), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return set_error(SSMX_ELAUNCH, "propagate launch: %s",
                          hipGetErrorString(e));
     return SSMX_OK;
+}
+
+// The exact-degree kernel for the common degree, else the general one.
+template <int SLOTS>
+int launch(const Params &p, hipStream_t stream)
+{
+    if constexpr (SLOTS <= 16) {
+        if (p.d.taylor == kUnrolledTaylor)
+            return launch_chain<SLOTS, kUnrolledTaylor>(p, stream);
+    }
+    return launch_chain<SLOTS, 0>(p, stream);
 }
 
 int validate(const ssmx_propagate_desc *d)

@@ -19,6 +19,7 @@
 namespace {
 
 using ssmx::ell_apply;
+using ssmx::kUnrolledMax;
 using ssmx::kUnrolledTaylor;
 using ssmx::kWave;
 using ssmx::node_time;
@@ -195,21 +196,28 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
     }
 }
 
-// The Taylor chain unrolled for the common degree (SLOTS <= 16), else the
-// runtime loop.
-template <int SLOTS, int NPAIR>
+// CHAIN > 0: the kernel variant launched for Taylor degree CHAIN, with the
+// chain unrolled for exactly that degree.  CHAIN = 0: the chain with
+// per-term exits for degrees up to kUnrolledMax (SLOTS <= 16), else the
+// runtime loop.  Separate kernels keep each chain's register allocation
+// apart (both in one kernel ran every degree 1-2 % slower).
+template <int SLOTS, int NPAIR, int CHAIN>
 __device__ __forceinline__ void queue_loop_any(const QParams &p, const double *vals, double *xbuf, int M, const int (&col)[SLOTS])
 {
-    if constexpr (SLOTS <= 16) {
-        if (p.d.taylor == kUnrolledTaylor) {
-            queue_loop<SLOTS, NPAIR, kUnrolledTaylor>(p, vals, xbuf, M, col);
-            return;
+    if constexpr (CHAIN > 0) {
+        queue_loop<SLOTS, NPAIR, CHAIN>(p, vals, xbuf, M, col);
+    } else {
+        if constexpr (SLOTS <= 16) {
+            if (p.d.taylor >= 1 && p.d.taylor <= kUnrolledMax) {
+                queue_loop<SLOTS, NPAIR, -kUnrolledMax>(p, vals, xbuf, M, col);
+                return;
+            }
         }
+        queue_loop<SLOTS, NPAIR, 0>(p, vals, xbuf, M, col);
     }
-    queue_loop<SLOTS, NPAIR, 0>(p, vals, xbuf, M, col);
 }
 
-template <int SLOTS>
+template <int SLOTS, int CHAIN>
 // At most 128 VGPRs: the grid is sized for four wavefronts per SIMD.
 __global__ __launch_bounds__(kWave *kMaxWavesPerBlock)
 __attribute__((amdgpu_waves_per_eu(4))) void queue_kernel(
@@ -253,21 +261,21 @@ __attribute__((amdgpu_waves_per_eu(4))) void queue_kernel(
         constexpr int kAll = (SLOTS - 2) / 2;
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
-                queue_loop_any<SLOTS, kAll>(p, lds, xbuf, M, col);
+                queue_loop_any<SLOTS, kAll, CHAIN>(p, lds, xbuf, M, col);
                 return;
             }
         }
         if (np >= 1) {
-            queue_loop_any<SLOTS, 1>(p, lds, xbuf, M, col);
+            queue_loop_any<SLOTS, 1, CHAIN>(p, lds, xbuf, M, col);
             return;
         }
     }
-    queue_loop_any<SLOTS, 0>(p, lds, xbuf, M, col);
+    queue_loop_any<SLOTS, 0, CHAIN>(p, lds, xbuf, M, col);
 }
 
-template <int SLOTS>
-int launch(const QParams &p, int n_waves, int waves_per_block,
-           hipStream_t stream)
+template <int SLOTS, int CHAIN>
+int launch_chain(const QParams &p, int n_waves, int waves_per_block,
+                 hipStream_t stream)
 {
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
     const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
@@ -280,20 +288,33 @@ int launch(const QParams &p, int n_waves, int waves_per_block,
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&queue_kernel<SLOTS>),
+            reinterpret_cast<const void *>(&queue_kernel<SLOTS, CHAIN>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
     const int blocks = (n_waves + waves_per_block - 1) / waves_per_block;
-    hipLaunchKernelGGL((queue_kernel<SLOTS>), dim3(blocks),
+    hipLaunchKernelGGL((queue_kernel<SLOTS, CHAIN>), dim3(blocks),
                        dim3(kWave * waves_per_block), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return set_error(SSMX_ELAUNCH, "queue launch: %s",
                          hipGetErrorString(e));
     return SSMX_OK;
+}
+
+// The exact-degree kernel for the common degree, else the general one.
+template <int SLOTS>
+int launch(const QParams &p, int n_waves, int waves_per_block,
+           hipStream_t stream)
+{
+    if constexpr (SLOTS <= 16) {
+        if (p.d.taylor == kUnrolledTaylor)
+            return launch_chain<SLOTS, kUnrolledTaylor>(
+                p, n_waves, waves_per_block, stream);
+    }
+    return launch_chain<SLOTS, 0>(p, n_waves, waves_per_block, stream);
 }
 
 int validate(const ssmx_propagate_desc *d, int n_waves, int waves_per_block)
