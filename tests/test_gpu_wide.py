@@ -2,12 +2,12 @@
 
 The 21-level NV + 14N model: n = 441 (coherent vectorisation, 7 wavefronts
 per instance, 2 instances per workgroup) and n = 171 (compressed, 3
-wavefronts, 5 instances per workgroup).
+wavefronts, 1 instance per workgroup, several workgroups per CU).
 
 * Short runs against the reference-algorithm oracle (numpy backend of the
   restated CPU path): every component of every output state, and two
   readout rows at once; 1e-10 relative to the largest |component|.
-  Instance counts leave idle slots in the last workgroup.
+  At n = 441 the odd count leaves an idle slot in the last workgroup.
 * Full-length sweep points (250,000 steps): the coherent and compressed
   vectorisations of the same physics give the same fluorescence (the
   compressed one drops coherences between incoherently coupled blocks,
