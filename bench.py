@@ -60,7 +60,8 @@ def parse():
     p.add_argument("--instances", type=int, default=None,
                    help="sweep points (default: the workload's count)")
     p.add_argument("--cpu-outputs", type=int, default=None,
-                   help="output intervals of the CPU-baseline sample")
+                   help="output intervals of the CPU-baseline sample "
+                        "(default: all of one sweep point)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device-to-device copy (HBM "
@@ -622,7 +623,9 @@ def main():
             args.hyperfine_adjoint_incoherent = inc
             result[key] = hyperfine_adjoint_leg(args)
     if not args.no_cpu:
-        outputs = args.cpu_outputs or (128 if wl.n > 20 else 999)
+        # one whole sweep point (999 outputs: about 12 s on 16 cores at
+        # n = 49)
+        outputs = args.cpu_outputs or wl.n_out
         result["cpu_baseline"] = cpu_baseline(wl, outputs)
         result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
     print(json.dumps(result))
