@@ -179,6 +179,52 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
     return y0 + y1;
 }
 
+// In a wave-uniform branch: keeps it a branch.  Without it the compiler
+// may compute the guarded work on every pass and select the result (a
+// v_cndmask per register), issue the headline kernels cannot spare.
+__device__ __forceinline__ void keep_branch()
+{
+    asm volatile("");
+}
+
+// The lane's row of A = sum_m f_m G_m over SLOTS slots from the staged
+// values `vals` ([m][slot / 2][lane][slot % 2], 64 lanes), f_m = lane
+// base + m of `factor`.  Each slot accumulates fma(f_m, v, .) from m = 0,
+// the first from 0.0 (the same bits as zeroing the row first, without the
+// moves).
+template <int SLOTS>
+__device__ __forceinline__ void assemble_row(double (&a)[SLOTS],
+                                             const double *vals,
+                                             double factor, int base,
+                                             int M, int lane)
+{
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    {
+        const double f = read_lane(factor, base);
+        const double2 *v = reinterpret_cast<const double2 *>(vals) + lane;
+#pragma unroll
+        for (int pr = 0; pr < SLOTP / 2; ++pr) {
+            const double2 vv = v[pr * kWave];
+            a[2 * pr] = fma(f, vv.x, 0.0);
+            if (2 * pr + 1 < SLOTS)
+                a[2 * pr + 1] = fma(f, vv.y, 0.0);
+        }
+    }
+    for (int m = 1; m < M; ++m) {
+        const double f = read_lane(factor, base + m);
+        const double2 *v =
+            reinterpret_cast<const double2 *>(vals + m * SLOTP * kWave) +
+            lane;
+#pragma unroll
+        for (int pr = 0; pr < SLOTP / 2; ++pr) {
+            const double2 vv = v[pr * kWave];
+            a[2 * pr] = fma(f, vv.x, a[2 * pr]);
+            if (2 * pr + 1 < SLOTS)
+                a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
+        }
+    }
+}
+
 // 1.0 / T (correctly rounded, as the compiler folds it) built by two
 // scalar moves at the point of use: as plain constants the compiler hoists
 // all of them out of the step loop, where they spill and come back through

@@ -18,7 +18,9 @@
 
 namespace {
 
+using ssmx::assemble_row;
 using ssmx::ell_apply;
+using ssmx::keep_branch;
 using ssmx::kUnrolledMax;
 using ssmx::kUnrolledTaylor;
 using ssmx::kWave;
@@ -96,7 +98,6 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
                                            const double *vals, double *xbuf,
                                            int M, const int (&col)[SLOTS])
 {
-    constexpr int SLOTP = SLOTS + (SLOTS & 1);
     const ssmx_propagate_desc &d = p.d;
     const int lane = threadIdx.x & (kWave - 1);
     const bool rotating = (d.flags & SSMX_ROTATING) != 0;
@@ -149,25 +150,9 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
             const double x_prev = x;
             for (int j = J - 1; j >= 0; --j) {
                 double a[SLOTS];
-#pragma unroll
-                for (int sl = 0; sl < SLOTS; ++sl)
-                    a[sl] = 0.0;
-                const int base = (b * J + j) * M;
-                for (int m = 0; m < M; ++m) {
-                    const double f = read_lane(factor, base + m);
-                    const double2 *v =
-                        reinterpret_cast<const double2 *>(vals +
-                                                          m * SLOTP * kWave) +
-                        lane;
-#pragma unroll
-                    for (int pr = 0; pr < SLOTP / 2; ++pr) {
-                        const double2 vv = v[pr * kWave];
-                        a[2 * pr] = fma(f, vv.x, a[2 * pr]);
-                        if (2 * pr + 1 < SLOTS)
-                            a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
-                    }
-                }
+                assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
                 if (d.substeps > 1) {
+                    keep_branch();
 #pragma unroll
                     for (int sl = 0; sl < SLOTS; ++sl)
                         a[sl] *= inv_sub;
