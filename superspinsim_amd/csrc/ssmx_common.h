@@ -225,6 +225,28 @@ __device__ __forceinline__ void assemble_row(double (&a)[SLOTS],
     }
 }
 
+// assemble_row with the MG generator rows held in registers (g[m][slot]):
+// the same operations in the same order, without the LDS reads.
+template <int SLOTS, int MG>
+__device__ __forceinline__ void assemble_row_reg(double (&a)[SLOTS],
+                                                 const double (&g)[MG][SLOTS],
+                                                 double factor, int base)
+{
+    {
+        const double f = read_lane(factor, base);
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; ++sl)
+            a[sl] = fma(f, g[0][sl], 0.0);
+    }
+#pragma unroll
+    for (int m = 1; m < MG; ++m) {
+        const double f = read_lane(factor, base + m);
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; ++sl)
+            a[sl] = fma(f, g[m][sl], a[sl]);
+    }
+}
+
 // 1.0 / T (correctly rounded, as the compiler folds it) built by two
 // scalar moves at the point of use: as plain constants the compiler hoists
 // all of them out of the step loop, where they spill and come back through

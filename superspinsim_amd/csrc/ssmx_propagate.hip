@@ -43,6 +43,7 @@
 namespace {
 
 using ssmx::assemble_row;
+using ssmx::assemble_row_reg;
 using ssmx::ell_apply;
 using ssmx::keep_branch;
 using ssmx::kUnrolledMax;
@@ -128,8 +129,9 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
 
 // One instance on one wavefront, after the kernel has staged the generator
 // values (`vals`, [m][slot / 2][lane][slot % 2]) and loaded the lane's ELL
-// columns.  NPAIR: leading slot pairs fetched whole (ell_apply).
-template <int SLOTS, int NPAIR, int TAY>
+// columns.  NPAIR: leading slot pairs fetched whole (ell_apply).  MG > 0:
+// the MG (= M) generator rows are read from LDS once into registers.
+template <int SLOTS, int NPAIR, int TAY, int MG = 0>
 __device__ __forceinline__ void propagate_instance(const Params &p,
                                                    const double *vals,
                                                    double *xbuf, int M,
@@ -145,6 +147,18 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
     const bool read_full = (d.flags & SSMX_READ_FULL) != 0;
     const bool read_int = (d.flags & SSMX_READ_INTEGRAL) != 0;
     const bool use_partner = d.partner != 0;
+
+    double g[MG > 0 ? MG : 1][SLOTS];
+    if constexpr (MG > 0) {
+        constexpr int SLOTP = SLOTS + (SLOTS & 1);
+#pragma unroll
+        for (int m = 0; m < MG; ++m)
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; ++sl)
+                g[m][sl] = vals[((m * (SLOTP / 2) + sl / 2) * kWave + lane) *
+                                    2 +
+                                (sl & 1)];
+    }
 
     // Per-instance views.
     Params q = p;
@@ -222,7 +236,10 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
             const double x_prev = x;
             for (int j = J - 1; j >= 0; --j) {
                 double a[SLOTS];
-                assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
+                if constexpr (MG > 0)
+                    assemble_row_reg<SLOTS, MG>(a, g, factor, (b * J + j) * M);
+                else
+                    assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
                 if (d.substeps > 1) {
                     keep_branch();
 #pragma unroll
@@ -278,11 +295,12 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
 // per-term exits for degrees up to kUnrolledMax (SLOTS <= 16), else the
 // runtime loop.  Separate kernels keep each chain's register allocation
 // apart (both in one kernel ran every degree 1-2 % slower).
-template <int SLOTS, int NPAIR, int CHAIN>
+template <int SLOTS, int NPAIR, int CHAIN, int MG>
 __device__ __forceinline__ void propagate_instance_any(const Params &p, const double *vals, double *xbuf, int M, int inst, const int (&col)[SLOTS])
 {
     if constexpr (CHAIN > 0) {
-        propagate_instance<SLOTS, NPAIR, CHAIN>(p, vals, xbuf, M, inst, col);
+        propagate_instance<SLOTS, NPAIR, CHAIN, MG>(p, vals, xbuf, M, inst,
+                                                   col);
     } else {
         if constexpr (SLOTS <= 16) {
             if (p.d.taylor >= 1 && p.d.taylor <= kUnrolledMax) {
@@ -294,7 +312,7 @@ __device__ __forceinline__ void propagate_instance_any(const Params &p, const do
     }
 }
 
-template <int SLOTS, int CHAIN>
+template <int SLOTS, int CHAIN, int MG>
 // At most 128 VGPRs: four wavefronts per SIMD (4096 instances fill the
 // device once).
 __global__ __launch_bounds__(kWave *kWavesPerBlock)
@@ -350,19 +368,19 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         constexpr int kAll = (SLOTS - 2) / 2;
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
-                propagate_instance_any<SLOTS, kAll, CHAIN>(p, lds, xbuf, M, inst, col);
+                propagate_instance_any<SLOTS, kAll, CHAIN, MG>(p, lds, xbuf, M, inst, col);
                 return;
             }
         }
         if (np >= 1) {
-            propagate_instance_any<SLOTS, 1, CHAIN>(p, lds, xbuf, M, inst, col);
+            propagate_instance_any<SLOTS, 1, CHAIN, MG>(p, lds, xbuf, M, inst, col);
             return;
         }
     }
-    propagate_instance_any<SLOTS, 0, CHAIN>(p, lds, xbuf, M, inst, col);
+    propagate_instance_any<SLOTS, 0, CHAIN, MG>(p, lds, xbuf, M, inst, col);
 }
 
-template <int SLOTS, int CHAIN>
+template <int SLOTS, int CHAIN, int MG>
 int launch_chain(const Params &p, hipStream_t stream)
 {
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
@@ -376,14 +394,14 @@ int launch_chain(const Params &p, hipStream_t stream)
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&propagate_kernel<SLOTS, CHAIN>),
+            reinterpret_cast<const void *>(&propagate_kernel<SLOTS, CHAIN, MG>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
     const int blocks = (p.d.n_inst + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((propagate_kernel<SLOTS, CHAIN>), dim3(blocks),
+    hipLaunchKernelGGL((propagate_kernel<SLOTS, CHAIN, MG>), dim3(blocks),
                        dim3(kWave * kWavesPerBlock), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -396,11 +414,17 @@ int launch_chain(const Params &p, hipStream_t stream)
 template <int SLOTS>
 int launch(const Params &p, hipStream_t stream)
 {
+    const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
+    if constexpr (SLOTS <= 8) {
+        // two generator rows in registers: no LDS reads in the assembly
+        if (p.d.taylor == kUnrolledTaylor && M == 2)
+            return launch_chain<SLOTS, kUnrolledTaylor, 2>(p, stream);
+    }
     if constexpr (SLOTS <= 16) {
         if (p.d.taylor == kUnrolledTaylor)
-            return launch_chain<SLOTS, kUnrolledTaylor>(p, stream);
+            return launch_chain<SLOTS, kUnrolledTaylor, 0>(p, stream);
     }
-    return launch_chain<SLOTS, 0>(p, stream);
+    return launch_chain<SLOTS, 0, 0>(p, stream);
 }
 
 int validate(const ssmx_propagate_desc *d)
