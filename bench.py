@@ -46,7 +46,7 @@ PEAK_HBM_GBS = 8000.0
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB copy
 # (fetch x2.0, write x1.0); the summary is kept with the profiles.
 MEASURED_TRAFFIC = {
-    ("odmr7-nv-n49", 4096, 99900): (1303.1 * 2.0 + 32166.5 * 1.0) * 1024,
+    ("odmr7-nv-n49", 4096, 99900): (3825.5 * 2.0 + 46499.1 * 1.0) * 1024,
 }
 
 
