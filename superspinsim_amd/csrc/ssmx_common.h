@@ -247,6 +247,33 @@ __device__ __forceinline__ void assemble_row_reg(double (&a)[SLOTS],
     }
 }
 
+// assemble_row with the first MG generator rows in registers (g[m][slot])
+// and rows MG..M-1 read from the staged values: the same operations in the
+// same order.
+template <int SLOTS, int MG>
+__device__ __forceinline__ void assemble_row_mix(double (&a)[SLOTS],
+                                                 const double (&g)[MG][SLOTS],
+                                                 const double *vals,
+                                                 double factor, int base,
+                                                 int M, int lane)
+{
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    assemble_row_reg<SLOTS, MG>(a, g, factor, base);
+    for (int m = MG; m < M; ++m) {
+        const double f = read_lane(factor, base + m);
+        const double2 *v =
+            reinterpret_cast<const double2 *>(vals + m * SLOTP * kWave) +
+            lane;
+#pragma unroll
+        for (int pr = 0; pr < SLOTP / 2; ++pr) {
+            const double2 vv = v[pr * kWave];
+            a[2 * pr] = fma(f, vv.x, a[2 * pr]);
+            if (2 * pr + 1 < SLOTS)
+                a[2 * pr + 1] = fma(f, vv.y, a[2 * pr + 1]);
+        }
+    }
+}
+
 // 1.0 / T (correctly rounded, as the compiler folds it) built by two
 // scalar moves at the point of use: as plain constants the compiler hoists
 // all of them out of the step loop, where they spill and come back through

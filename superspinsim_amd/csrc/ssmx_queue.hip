@@ -19,6 +19,7 @@
 namespace {
 
 using ssmx::assemble_row;
+using ssmx::assemble_row_mix;
 using ssmx::ell_apply;
 using ssmx::keep_branch;
 using ssmx::kUnrolledMax;
@@ -92,8 +93,9 @@ __device__ double step_factor_q(const QParams &p, const ssmx_wave *wv,
 
 // The queue loop of one wavefront, after the kernel has staged the
 // generator values (`vals`) and loaded the lane's ELL columns.  NPAIR: leading
-// slot pairs fetched whole (ell_apply).
-template <int SLOTS, int NPAIR, int TAY>
+// slot pairs fetched whole (ell_apply).  MG > 0: the first MG generator
+// rows are read from LDS once into registers, the others at each exponent.
+template <int SLOTS, int NPAIR, int TAY, int MG = 0>
 __device__ __forceinline__ void queue_loop(const QParams &p,
                                            const double *vals, double *xbuf,
                                            int M, const int (&col)[SLOTS])
@@ -105,6 +107,17 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
     const int J = d.n_exp;
     const bool use_partner = d.partner != 0;
     const double inv_lane = kInvQ[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+    double g[MG > 0 ? MG : 1][SLOTS];
+    if constexpr (MG > 0) {
+        constexpr int SLOTP = SLOTS + (SLOTS & 1);
+#pragma unroll
+        for (int m = 0; m < MG; ++m)
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; ++sl)
+                g[m][sl] = vals[((m * (SLOTP / 2) + sl / 2) * kWave + lane) *
+                                    2 +
+                                (sl & 1)];
+    }
     double eig_own = 0.0, eig_part = 0.0;
     if (rotating) {
         eig_own = p.eig[lane];
@@ -150,7 +163,11 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
             const double x_prev = x;
             for (int j = J - 1; j >= 0; --j) {
                 double a[SLOTS];
-                assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
+                if constexpr (MG > 0)
+                    assemble_row_mix<SLOTS, MG>(a, g, vals, factor,
+                                                (b * J + j) * M, M, lane);
+                else
+                    assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
                 if (d.substeps > 1) {
                     keep_branch();
 #pragma unroll
@@ -186,11 +203,11 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
 // per-term exits for degrees up to kUnrolledMax (SLOTS <= 16), else the
 // runtime loop.  Separate kernels keep each chain's register allocation
 // apart (both in one kernel ran every degree 1-2 % slower).
-template <int SLOTS, int NPAIR, int CHAIN>
+template <int SLOTS, int NPAIR, int CHAIN, int MG>
 __device__ __forceinline__ void queue_loop_any(const QParams &p, const double *vals, double *xbuf, int M, const int (&col)[SLOTS])
 {
     if constexpr (CHAIN > 0) {
-        queue_loop<SLOTS, NPAIR, CHAIN>(p, vals, xbuf, M, col);
+        queue_loop<SLOTS, NPAIR, CHAIN, MG>(p, vals, xbuf, M, col);
     } else {
         if constexpr (SLOTS <= 16) {
             if (p.d.taylor >= 1 && p.d.taylor <= kUnrolledMax) {
@@ -202,7 +219,7 @@ __device__ __forceinline__ void queue_loop_any(const QParams &p, const double *v
     }
 }
 
-template <int SLOTS, int CHAIN>
+template <int SLOTS, int CHAIN, int MG>
 // At most 128 VGPRs: the grid is sized for four wavefronts per SIMD.
 __global__ __launch_bounds__(kWave *kMaxWavesPerBlock)
 __attribute__((amdgpu_waves_per_eu(4))) void queue_kernel(
@@ -246,19 +263,19 @@ __attribute__((amdgpu_waves_per_eu(4))) void queue_kernel(
         constexpr int kAll = (SLOTS - 2) / 2;
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
-                queue_loop_any<SLOTS, kAll, CHAIN>(p, lds, xbuf, M, col);
+                queue_loop_any<SLOTS, kAll, CHAIN, MG>(p, lds, xbuf, M, col);
                 return;
             }
         }
         if (np >= 1) {
-            queue_loop_any<SLOTS, 1, CHAIN>(p, lds, xbuf, M, col);
+            queue_loop_any<SLOTS, 1, CHAIN, MG>(p, lds, xbuf, M, col);
             return;
         }
     }
-    queue_loop_any<SLOTS, 0, CHAIN>(p, lds, xbuf, M, col);
+    queue_loop_any<SLOTS, 0, CHAIN, MG>(p, lds, xbuf, M, col);
 }
 
-template <int SLOTS, int CHAIN>
+template <int SLOTS, int CHAIN, int MG>
 int launch_chain(const QParams &p, int n_waves, int waves_per_block,
                  hipStream_t stream)
 {
@@ -273,14 +290,14 @@ int launch_chain(const QParams &p, int n_waves, int waves_per_block,
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&queue_kernel<SLOTS, CHAIN>),
+            reinterpret_cast<const void *>(&queue_kernel<SLOTS, CHAIN, MG>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
     const int blocks = (n_waves + waves_per_block - 1) / waves_per_block;
-    hipLaunchKernelGGL((queue_kernel<SLOTS, CHAIN>), dim3(blocks),
+    hipLaunchKernelGGL((queue_kernel<SLOTS, CHAIN, MG>), dim3(blocks),
                        dim3(kWave * waves_per_block), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -294,12 +311,19 @@ template <int SLOTS>
 int launch(const QParams &p, int n_waves, int waves_per_block,
            hipStream_t stream)
 {
-    if constexpr (SLOTS <= 16) {
-        if (p.d.taylor == kUnrolledTaylor)
-            return launch_chain<SLOTS, kUnrolledTaylor>(
+    const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
+    if constexpr (SLOTS <= 8) {
+        // two generator rows in registers, the others from LDS
+        if (p.d.taylor == kUnrolledTaylor && M >= 2)
+            return launch_chain<SLOTS, kUnrolledTaylor, 2>(
                 p, n_waves, waves_per_block, stream);
     }
-    return launch_chain<SLOTS, 0>(p, n_waves, waves_per_block, stream);
+    if constexpr (SLOTS <= 16) {
+        if (p.d.taylor == kUnrolledTaylor)
+            return launch_chain<SLOTS, kUnrolledTaylor, 0>(
+                p, n_waves, waves_per_block, stream);
+    }
+    return launch_chain<SLOTS, 0, 0>(p, n_waves, waves_per_block, stream);
 }
 
 int validate(const ssmx_propagate_desc *d, int n_waves, int waves_per_block)
