@@ -43,6 +43,7 @@
 namespace {
 
 using ssmx::assemble_row;
+using ssmx::assemble_row_mix;
 using ssmx::assemble_row_reg;
 using ssmx::ell_apply;
 using ssmx::keep_branch;
@@ -130,7 +131,8 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
 // One instance on one wavefront, after the kernel has staged the generator
 // values (`vals`, [m][slot / 2][lane][slot % 2]) and loaded the lane's ELL
 // columns.  NPAIR: leading slot pairs fetched whole (ell_apply).  MG > 0:
-// the MG (= M) generator rows are read from LDS once into registers.
+// the MG (= M) generator rows are read from LDS once into registers;
+// MG < 0: the first -MG rows are, the others are read at each exponent.
 template <int SLOTS, int NPAIR, int TAY, int MG = 0>
 __device__ __forceinline__ void propagate_instance(const Params &p,
                                                    const double *vals,
@@ -148,11 +150,12 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
     const bool read_int = (d.flags & SSMX_READ_INTEGRAL) != 0;
     const bool use_partner = d.partner != 0;
 
-    double g[MG > 0 ? MG : 1][SLOTS];
-    if constexpr (MG > 0) {
+    constexpr int MGR = MG < 0 ? -MG : MG;
+    double g[MGR > 0 ? MGR : 1][SLOTS];
+    if constexpr (MGR > 0) {
         constexpr int SLOTP = SLOTS + (SLOTS & 1);
 #pragma unroll
-        for (int m = 0; m < MG; ++m)
+        for (int m = 0; m < MGR; ++m)
 #pragma unroll
             for (int sl = 0; sl < SLOTS; ++sl)
                 g[m][sl] = vals[((m * (SLOTP / 2) + sl / 2) * kWave + lane) *
@@ -238,6 +241,9 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                 double a[SLOTS];
                 if constexpr (MG > 0)
                     assemble_row_reg<SLOTS, MG>(a, g, factor, (b * J + j) * M);
+                else if constexpr (MG < 0)
+                    assemble_row_mix<SLOTS, MGR>(a, g, vals, factor,
+                                                 (b * J + j) * M, M, lane);
                 else
                     assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
                 if (d.substeps > 1) {
@@ -419,6 +425,9 @@ int launch(const Params &p, hipStream_t stream)
         // two generator rows in registers: no LDS reads in the assembly
         if (p.d.taylor == kUnrolledTaylor && M == 2)
             return launch_chain<SLOTS, kUnrolledTaylor, 2>(p, stream);
+        // more generators: the first two rows in registers
+        if (p.d.taylor == kUnrolledTaylor && M > 2)
+            return launch_chain<SLOTS, kUnrolledTaylor, -2>(p, stream);
     }
     if constexpr (SLOTS <= 16) {
         if (p.d.taylor == kUnrolledTaylor)
