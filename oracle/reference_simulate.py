@@ -263,25 +263,14 @@ def generate_simulator(sampler, generators, vectorisation_map=None,
                 d_step if use_rotating else None,
                 s_step if use_rotating else None, _exponentials)
 
-        eye = np.eye(n)
         if use_rotating:
-            full = te + eye if use_residual else te
-            te = vectors_real @ full @ inv_vectors_real
-            if use_residual:
-                te = te - eye
+            te = rotating_sandwich(te, vectors_real, inv_vectors_real,
+                                   use_residual)
         if use_kernel:
-            P = np.asarray(image_projection, dtype=_f64)
-            full = te + np.eye(te.shape[-1]) if use_residual else te
-            te = P @ full @ P.T
-            te = te + (np.eye(P.shape[0]) - P @ P.T)
-            if use_residual:
-                te = te - np.eye(P.shape[0])
+            te = kernel_sandwich(te, image_projection, use_residual)
         if use_unitary:
-            te = _unitary_lift(te, n, use_residual)
-            full = te + np.eye(te.shape[-1]) if use_residual else te
-            te = np.asarray(elimination) @ full @ np.asarray(duplication)
-            if use_residual:
-                te = te - np.eye(te.shape[-1])
+            te = unitary_sandwich(te, n, elimination, duplication,
+                                  use_residual)
 
         if use_residual:
             flat = rho0[None] + te @ rho0
@@ -291,6 +280,42 @@ def generate_simulator(sampler, generators, vectorisation_map=None,
         return time, density
 
     return simulate
+
+
+def _sandwich(left, te, right, residual):
+    """``L (I + TE) R - I`` (residual) or ``L TE R``
+    (``_apply_global_sandwich_run`` :1023-1171)."""
+    if not residual:
+        return left @ te @ right
+    full = te + np.eye(te.shape[-1])
+    return left @ full @ right - np.eye(left.shape[0])
+
+
+def rotating_sandwich(te, vectors_real, inv_vectors_real, residual):
+    """Leave the rotating frame: ``V (I + TE) V^-1 - I`` (:2030-2038)."""
+    return _sandwich(np.asarray(vectors_real, dtype=_f64), te,
+                     np.asarray(inv_vectors_real, dtype=_f64), residual)
+
+
+def kernel_sandwich(te, image_projection, residual):
+    """Leave the equivalence class: ``P (I + TE') P^T + (I - P P^T) - I``.
+
+    ``_apply_global_sandwich_run`` with (P, P^T) then
+    ``_apply_global_addition_run`` of ``I - P P^T`` (:2043-2067, addition
+    :1173-1187, kernel projection :1789-1797).  The kernel part keeps its
+    component of rho0 unchanged, since every generator annihilates it.
+    """
+    P = np.asarray(image_projection, dtype=_f64)
+    te = _sandwich(P, te, P.T, residual)
+    return te + (np.eye(P.shape[0]) - P @ P.T)
+
+
+def unitary_sandwich(te, n, elimination, duplication, residual):
+    """Unitary mode: lift U to ``conj(U) (x) U`` in the real interleaved
+    form, then ``E (I + lift) D - I`` (:1394-1504, 2074-2105)."""
+    lifted = _unitary_lift(te, n, residual)
+    return _sandwich(np.asarray(elimination, dtype=_f64), lifted,
+                     np.asarray(duplication, dtype=_f64), residual)
 
 
 def _unitary_lift(te, n, residual):
@@ -390,4 +415,5 @@ def exact_step_count(time_start, time_end, time_step, fine):
 
 
 __all__ = ["generate_simulator", "rotating_precompute", "eig_step",
-           "flatten", "unflatten", "exact_step_count", "math"]
+           "flatten", "unflatten", "rotating_sandwich", "kernel_sandwich",
+           "unitary_sandwich", "exact_step_count", "math"]

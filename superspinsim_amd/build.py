@@ -1,22 +1,27 @@
-"""Compile the HIP library ``lib/libssmx.so`` for gfx950 (in-tree)."""
+"""Compile the HIP library ``lib/libssmx.so`` for gfx950 (in-tree).
+
+Each source is compiled to its own object in parallel (the kernels of one
+file never call device code of another), then the objects are linked into
+one shared library.
+"""
 
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
+OBJ_DIR = os.path.join(LIB_DIR, "_obj")
 SOURCES = ["ssmx_propagate.hip", "ssmx_adjoint.hip", "ssmx_queue.hip",
-           "ssmx_wide.hip",
-           "ssmx_adjoint_wide.hip"]
+           "ssmx_wide.hip", "ssmx_adjoint_wide.hip"]
 
 # -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32 (see DESIGN notes on
 # packed FP32 on this hardware); the host pass warns that it does not know
 # the feature, which is harmless.
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-               "-shared", "-Xclang", "-target-feature", "-Xclang",
-               "-packed-fp32-ops"]
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def hipcc():
@@ -27,13 +32,34 @@ def hipcc():
     raise FileNotFoundError("hipcc not found")
 
 
+def _jobs():
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = os.cpu_count() or 1
+    return max(1, min(len(SOURCES), cpus, 16))
+
+
 def build_library(verbose=False):
     """Compile every HIP source into ``lib/libssmx.so``; returns its path."""
-    os.makedirs(LIB_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
     out = os.path.join(LIB_DIR, "libssmx.so")
     tmp = out + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + \
-        [os.path.join(CSRC, s) for s in SOURCES]
+    cc = hipcc()
+    sources = list(SOURCES)
+
+    def compile_one(src):
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = [cc] + HIPCC_FLAGS + ["-c", "-o", obj,
+                                    os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=_jobs()) as pool:
+        objs = list(pool.map(compile_one, sources))
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

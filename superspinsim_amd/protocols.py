@@ -128,7 +128,12 @@ class ProtocolBatch:
         torch = _torch()
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
-        self.counter.zero_()
+            self.counter.zero_()
+        else:
+            # the reset (a fill kernel) must run on the launch's stream, or
+            # the kernel could read the previous launch's final count
+            with torch.cuda.stream(torch.cuda.ExternalStream(stream)):
+                self.counter.zero_()
         desc = self._desc()
         prop = self.prop
         rc = _lib.load().ssmx_propagate_queue(

@@ -118,3 +118,18 @@ def test_autotuner_choice_reproduces(wl, queued, tmp_path, monkeypatch):
     assert np.max(np.abs(got - out) / np.abs(out)) < 1e-10
     again = autotune.tune(b, trial_steps=1000, repeats=1)
     assert again["cached"]
+
+
+def test_launch_on_explicit_stream(wl, queued):
+    """The counter reset runs on the launch's own stream: two launches in a
+    row on a side stream both drain the whole queue."""
+    import torch
+    b, out = queued
+    side = torch.cuda.Stream()
+    for k in range(2):
+        b.out.fill_(np.nan)
+        side.wait_stream(torch.cuda.current_stream())
+        b.launch(stream=side.cuda_stream)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        assert np.array_equal(b.out.cpu().numpy()[:, 0], out), k

@@ -125,3 +125,19 @@ def test_rotating_frame_converges_to_lab_frame():
     lab = _driven_qubit(256)
     rot = _driven_qubit(256, rotating=True)
     assert np.max(np.abs(lab - rot)) < 1e-9
+
+
+def test_vectorised_waveform_sampler_matches_waveform():
+    """oracle.waves evaluates the same coefficient formula as Waveform."""
+    from oracle.waves import vectorised_sampler
+    from superspinsim_amd.sweep import Waveform
+    w = Waveform(4, base=[1.0, 0.0, 0.5, 0.0])
+    w.tone(1, 2e-4, 2.87e9, t_start=1e-9, t_end=3e-9, phase=0.3)
+    w.add(1, 3e-9, 5e-9, 1e-4)
+    w.add(3, 0.0, 2e-9, -3e-4, omega=1e8)
+    rng = np.random.default_rng(0)
+    ts = np.concatenate([rng.uniform(-1e-9, 6e-9, 200),
+                         [0.0, 1e-9, 2e-9, 3e-9, 5e-9]])
+    got = vectorised_sampler(w)(ts)
+    want = np.array([w(t) for t in ts])
+    assert np.array_equal(got, want)
