@@ -102,6 +102,12 @@ def load():
         raise ImportError(
             f"{LIB_PATH} not found: build the HIP library first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
+    # torch first: its bundled HIP runtime (soname libamdhip64.so.7) then
+    # satisfies this library's dependency.  Loaded the other way round, the
+    # process would map two HIP runtimes (torch asks for "libamdhip64.so",
+    # which does not match the soname already loaded), and launches from
+    # this library would find no device.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
     lib.ssmx_propagate.argtypes = [ctypes.POINTER(PropagateDesc)] + [vp] * 13

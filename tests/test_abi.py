@@ -172,3 +172,17 @@ def test_adjoint_null_pointers_are_rejected(lib):
 
 def test_version(lib):
     assert lib.ssmx_version() >= 100
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """Loading the library before anything imports torch must not map a
+    second HIP runtime next to torch's (launches would find no device)."""
+    import sys
+    code = ("import re, sys; sys.path.insert(0, %r)\n"
+            "from superspinsim_amd import _lib\n_lib.load()\nimport torch\n"
+            "maps = open('/proc/self/maps').read()\n"
+            "print(sorted(set(re.findall(r'\\S*libamdhip64\\S*', maps))))"
+            % ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True,
+                         text=True, check=True).stdout.strip()
+    assert out.count("libamdhip64") == 1, out
