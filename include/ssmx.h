@@ -163,6 +163,57 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
                    void *stream);
 
 /*
+ * The reference's non-residual exponential (use_residual=False).
+ *
+ * Each exponential is (I + A / scale)^(2^squarings) with scale = 4^R and
+ * squarings = 2R, formed densely: A_j is assembled as ssmx_propagate does,
+ * divided by `scale`, the identity added, then squared with plain products
+ * (k ascending in every dot product, fused multiply-adds).  In FP64 the
+ * diagonal 1 + a / 4^R rounds to 1 for small |a|, so damping terms are
+ * lost exactly as the reference loses them.  The state is propagated
+ * through the resulting matrices (exponential J-1 first, then in the
+ * rotating frame the eigen-step exp(L0 h)) and written at every output.
+ *
+ * gens       non-rotating: double [n_gen][n][n]; rotating: [n_node][n_gen]
+ *            [n][n] (node-wise interaction-picture generators), row-major
+ * cf_weights double [n_exp * n_node]
+ * coef       double [n_inst? * n_out * fine * n_node * n_gen] ordered
+ *            (instance, i, r, s, g) as the ssmx_propagate table mode
+ * rho0       double [n_inst * n]
+ * eig        rotating: double [2 n_doubles + n_singles]: (cos, sin) factors
+ *            (d0, d1) of each 2x2 block acting on rows (2y, 2y + 1) as
+ *            [[d0, -d1], [d1, d0]], then the factor of each single row
+ *            2 n_doubles + y; NULL otherwise
+ * out        double [n_inst * n_out * n] states at every output
+ *
+ * Replaces: generate_simulator.py:534-541 (scaling, + I) and :706-724,
+ * :882-897 (squarings) with use_residual=False, within the kernel sequence
+ * :1945-2116.
+ */
+typedef struct ssmx_dense_desc {
+    int32_t n;          /* superoperator dimension, 1..64 */
+    int32_t n_inst;     /* one workgroup each */
+    int32_t n_gen;      /* G, 1..8 */
+    int32_t n_exp;      /* J, 1..6 */
+    int32_t n_node;     /* S, 1..3 */
+    int32_t fine;       /* F propagator steps per output */
+    int32_t n_out;      /* outputs per instance */
+    int32_t squarings;  /* 2R, 0..1024 */
+    int32_t flags;      /* 0 or SSMX_ROTATING */
+    int32_t n_doubles;  /* rotating: 2x2 eigen-step blocks */
+    int32_t n_singles;  /* rotating: 1x1 eigen-step blocks */
+    int32_t reserved;   /* 0 */
+    int64_t coef_inst_stride; /* doubles between instances; 0 = shared */
+    double h;           /* propagator step dt / F (s) */
+    double scale;       /* 4^R */
+} ssmx_dense_desc;
+
+int ssmx_propagate_dense(const ssmx_dense_desc *desc, const double *gens,
+                         const double *cf_weights, const double *coef,
+                         const double *rho0, const double *eig, double *out,
+                         void *stream);
+
+/*
  * Persistent work-queue propagation of protocols with uneven lengths.
  *
  * Instance i runs steps[i] propagator steps (node times as ssmx_propagate

@@ -67,6 +67,20 @@ class AdjointDesc(ctypes.Structure):
     ]
 
 
+class DenseDesc(ctypes.Structure):
+    """Mirror of ``ssmx_dense_desc`` (include/ssmx.h)."""
+    _fields_ = [
+        ("n", ctypes.c_int32), ("n_inst", ctypes.c_int32),
+        ("n_gen", ctypes.c_int32), ("n_exp", ctypes.c_int32),
+        ("n_node", ctypes.c_int32), ("fine", ctypes.c_int32),
+        ("n_out", ctypes.c_int32), ("squarings", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("n_doubles", ctypes.c_int32),
+        ("n_singles", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("coef_inst_stride", ctypes.c_int64),
+        ("h", ctypes.c_double), ("scale", ctypes.c_double),
+    ]
+
+
 class Segment(ctypes.Structure):
     """Mirror of ``ssmx_segment``."""
     _fields_ = [("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
@@ -82,7 +96,8 @@ class Wave(ctypes.Structure):
 
 
 #: every symbol include/ssmx.h declares
-EXPORTS = ("ssmx_propagate", "ssmx_propagate_queue", "ssmx_adjoint",
+EXPORTS = ("ssmx_propagate", "ssmx_propagate_dense", "ssmx_propagate_queue",
+           "ssmx_adjoint",
            "ssmx_adjoint_workspace_bytes", "ssmx_last_error", "ssmx_version",
            "ssmx_device_count")
 
@@ -116,6 +131,9 @@ def load():
         [ctypes.POINTER(PropagateDesc)] + [vp] * 13 +
         [ctypes.c_int32, ctypes.c_int32, vp, vp])
     lib.ssmx_propagate_queue.restype = ctypes.c_int
+    lib.ssmx_propagate_dense.argtypes = [ctypes.POINTER(DenseDesc)] + \
+        [vp] * 7
+    lib.ssmx_propagate_dense.restype = ctypes.c_int
     lib.ssmx_adjoint.argtypes = [ctypes.POINTER(AdjointDesc)] + [vp] * 17
     lib.ssmx_adjoint.restype = ctypes.c_int
     lib.ssmx_adjoint_workspace_bytes.argtypes = [

@@ -22,16 +22,20 @@ The reference builds each interval's operator by 2R residual squarings and a
 serial prefix product; here the state is propagated with a Taylor action of
 the same exponentials (degree chosen per call from a norm bound, truncation
 error below 1e-16 per exponential), which equals the reference in exact
-arithmetic.  ``use_residual=False`` selects the same exponentials; the
-reference's non-residual rounding loss (``1 + A / 4^R`` rounding the diagonal
-to 1) is not reproduced.
+arithmetic.  ``use_residual=False`` runs the reference's own non-residual
+arithmetic instead (``ssmx_propagate_dense``): every exponential is formed as
+``(I + A / 4^R)`` squared 2R times, whose FP64 rounding drops the diagonal
+(damping) terms exactly as the reference does (generate_simulator.py:534-541,
+706-724).
 """
+
+import ctypes
 
 import numpy as np
 
-from superspinsim_amd import quadratures
+from superspinsim_amd import _lib, quadratures
 from superspinsim_amd.propagator import (
-    Propagator, device_tensor, eig_lane_factors, rotating_tables)
+    Propagator, _ptr, device_tensor, eig_lane_factors, rotating_tables)
 
 _f64 = np.float64
 
@@ -162,6 +166,16 @@ def generate_simulator(
 
         coef = tabulate(sampler, gens.shape[0], N, F, nodes, time_start,
                         time_step)
+        if not use_residual:
+            states = _nonresidual_states(
+                gens, weights, coef, rho_flat, h, number_of_quartic_repeats,
+                use_rotating, doubles, singles, nodes, time_step, F,
+                inv_vectors_real, use_kernel, image_projection, use_unitary)
+            return time, _finish(states, n, rho_flat, vmap, use_hermitian,
+                                 hilbert, use_rotating, vectors_real,
+                                 inv_vectors_real, use_kernel,
+                                 image_projection, use_unitary, elimination,
+                                 duplication)
         # Generators whose coefficient never changes are summed into one
         # matrix with coefficient 1; all-zero coefficients are dropped.
         cols = coef.reshape(-1, coef.shape[-1])
@@ -207,15 +221,10 @@ def generate_simulator(
             print(f"Taylor degree {m}, {s} substeps, {prop.ell.slots} ELL "
                   f"slots, nnz {prop.ell.nnz}")
 
-        if use_unitary:
-            # Propagate every column of the identity: the unitary's real
-            # interleaved form at each output (operator mode).
-            x0 = np.eye(n, dtype=_f64)
-        else:
-            x0 = rho_flat[None, :] if not use_kernel else \
-                (image_projection.T @ rho_flat)[None, :]
-            if use_rotating:
-                x0 = x0 @ inv_vectors_real.T
+        # unitary mode propagates every column of the identity: the
+        # unitary's real interleaved form at each output (operator mode)
+        x0 = _initial_rows(rho_flat, n, use_rotating, inv_vectors_real,
+                           use_kernel, image_projection, use_unitary)
         n_inst = x0.shape[0]
         d_rho = device_tensor(x0)
         d_coef = device_tensor(coef)
@@ -225,22 +234,89 @@ def generate_simulator(
                        read="full")
         torch.cuda.synchronize()
         states = out.cpu().numpy()                           # (I, N, n)
-
-        if use_unitary:
-            return time, _unitary_density(states, n, rho_flat, elimination,
-                                          duplication, vmap, use_hermitian,
-                                          hilbert, use_rotating,
-                                          vectors_real, inv_vectors_real)
-        x = states[0]                                        # (N, n)
-        if use_rotating:
-            x = x @ vectors_real.T
-        if use_kernel:
-            base = rho_flat - image_projection @ (image_projection.T
-                                                  @ rho_flat)
-            x = x @ image_projection.T + base[None, :]
-        return time, _unflatten(x, vmap, use_hermitian, hilbert)
+        return time, _finish(states, n, rho_flat, vmap, use_hermitian,
+                             hilbert, use_rotating, vectors_real,
+                             inv_vectors_real, use_kernel, image_projection,
+                             use_unitary, elimination, duplication)
 
     return simulate
+
+
+def _initial_rows(rho_flat, n, use_rotating, inv_vectors_real, use_kernel,
+                  image_projection, use_unitary):
+    """States the device propagates: the identity's columns (unitary mode,
+    operator form) or rho0 projected onto the image (kernel mode) and moved
+    into the eigenbasis (rotating frame)."""
+    if use_unitary:
+        x0 = np.eye(n, dtype=_f64)
+    else:
+        x0 = rho_flat[None, :] if not use_kernel else \
+            (image_projection.T @ rho_flat)[None, :]
+    if use_rotating and not use_unitary:
+        x0 = x0 @ inv_vectors_real.T
+    return np.ascontiguousarray(x0)
+
+
+def _finish(states, n, rho_flat, vmap, use_hermitian, hilbert, use_rotating,
+            vectors_real, inv_vectors_real, use_kernel, image_projection,
+            use_unitary, elimination, duplication):
+    """Device states (I, N, n) to densities: the global sandwiches of the
+    reference (rotating V, kernel P, unitary lift; :2030-2105) and the
+    unflattening (:2156-2191)."""
+    if use_unitary:
+        return _unitary_density(states, n, rho_flat, elimination,
+                                duplication, vmap, use_hermitian, hilbert,
+                                use_rotating, vectors_real, inv_vectors_real)
+    x = states[0]                                            # (N, n)
+    if use_rotating:
+        x = x @ vectors_real.T
+    if use_kernel:
+        base = rho_flat - image_projection @ (image_projection.T @ rho_flat)
+        x = x @ image_projection.T + base[None, :]
+    return _unflatten(x, vmap, use_hermitian, hilbert)
+
+
+def _nonresidual_states(gens, weights, coef, rho_flat, h, repeats,
+                        use_rotating, doubles, singles, nodes, time_step, F,
+                        inv_vectors_real, use_kernel, image_projection,
+                        use_unitary):
+    """States (I, N, n) of the reference's non-residual arithmetic on the
+    device: every generator and coefficient as tabulated (no folding, so
+    the assembly sums the same terms in the same order as the reference)."""
+    torch = __import__("torch")
+    n = gens.shape[1]
+    if n > _lib.MAX_N:
+        raise NotImplementedError(
+            f"use_residual=False is supported for n <= {_lib.MAX_N}")
+    x0 = _initial_rows(rho_flat, n, use_rotating, inv_vectors_real,
+                       use_kernel, image_projection, use_unitary)
+    N, _, S, G = coef.shape
+    flags, nd, ns, eig = 0, 0, 0, None
+    mats = gens
+    if use_rotating:
+        mats, d_step, s_step = rotating_tables(gens, doubles, singles, nodes,
+                                               time_step, F, residual=False)
+        flags = _lib.SSMX_ROTATING
+        nd, ns = d_step.shape[0], s_step.size
+        eig = device_tensor(np.concatenate([d_step.ravel(), s_step]))
+    d_gens = device_tensor(np.ascontiguousarray(mats, dtype=_f64))
+    d_w = device_tensor(np.ascontiguousarray(weights, dtype=_f64))
+    d_coef = device_tensor(np.ascontiguousarray(coef, dtype=_f64))
+    d_rho = device_tensor(x0)
+    n_inst = x0.shape[0]
+    out = torch.empty((n_inst, N, n), dtype=torch.float64, device="cuda")
+    desc = _lib.DenseDesc(
+        n=n, n_inst=n_inst, n_gen=G, n_exp=weights.shape[0], n_node=S,
+        fine=F, n_out=N, squarings=2 * int(repeats), flags=flags,
+        n_doubles=nd, n_singles=ns, reserved=0, coef_inst_stride=0, h=h,
+        scale=4.0 ** int(repeats))
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.load().ssmx_propagate_dense(
+        ctypes.byref(desc), _ptr(d_gens), _ptr(d_w), _ptr(d_coef),
+        _ptr(d_rho), _ptr(eig) if eig is not None else None, _ptr(out),
+        ctypes.c_void_p(stream)))
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
 
 
 def _unitary_density(columns, n, rho_flat, elimination, duplication, vmap,

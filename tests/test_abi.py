@@ -51,7 +51,7 @@ def test_struct_layout_matches_header(tmp_path):
 #include <stdio.h>
 #include "ssmx.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n",
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n",
          sizeof(ssmx_propagate_desc),
          offsetof(ssmx_propagate_desc, coef_inst_stride),
          offsetof(ssmx_propagate_desc, h), offsetof(ssmx_propagate_desc, dt),
@@ -61,7 +61,9 @@ int main(void) {
          sizeof(ssmx_adjoint_desc),
          offsetof(ssmx_adjoint_desc, grad_gen),
          offsetof(ssmx_adjoint_desc, coef_inst_stride),
-         offsetof(ssmx_adjoint_desc, dt));
+         offsetof(ssmx_adjoint_desc, dt), sizeof(ssmx_dense_desc),
+         offsetof(ssmx_dense_desc, coef_inst_stride),
+         offsetof(ssmx_dense_desc, scale));
   return 0;
 }
 ''')
@@ -76,7 +78,10 @@ int main(void) {
             D.dt.offset, ctypes.sizeof(_lib.Segment),
             ctypes.sizeof(_lib.Wave), _lib.Wave.seg_count.offset,
             D.ckpt_every.offset, ctypes.sizeof(A), A.grad_gen.offset,
-            A.coef_inst_stride.offset, A.dt.offset]
+            A.coef_inst_stride.offset, A.dt.offset,
+            ctypes.sizeof(_lib.DenseDesc),
+            _lib.DenseDesc.coef_inst_stride.offset,
+            _lib.DenseDesc.scale.offset]
     assert got == mine
 
 
@@ -186,3 +191,32 @@ def test_one_hip_runtime_whatever_the_import_order():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True,
                          text=True, check=True).stdout.strip()
     assert out.count("libamdhip64") == 1, out
+
+
+def _ddesc(**kw):
+    base = dict(n=4, n_inst=1, n_gen=1, n_exp=1, n_node=1, fine=1, n_out=1,
+                squarings=70, flags=0, n_doubles=0, n_singles=0, reserved=0,
+                coef_inst_stride=0, h=1e-11, scale=4.0 ** 35)
+    base.update(kw)
+    return _lib.DenseDesc(**base)
+
+
+@pytest.mark.parametrize("bad", [
+    dict(n=0), dict(n=65), dict(n_inst=0), dict(n_gen=9), dict(n_exp=7),
+    dict(n_node=4), dict(fine=0), dict(n_out=-1), dict(squarings=-1),
+    dict(scale=0.0), dict(flags=4), dict(n_doubles=1),
+    dict(flags=_lib.SSMX_ROTATING, n_doubles=2, n_singles=1),
+    dict(reserved=1), dict(h=0.0), dict(coef_inst_stride=-1),
+])
+def test_invalid_dense_descriptor_is_rejected(lib, bad):
+    d = _ddesc(**bad)
+    dummy = ctypes.c_void_p(8)
+    rc = lib.ssmx_propagate_dense(ctypes.byref(d), *([dummy] * 7))
+    assert rc == -1
+    assert lib.ssmx_last_error().decode()
+
+
+def test_dense_null_pointers_are_rejected(lib):
+    d = _ddesc()
+    assert lib.ssmx_propagate_dense(ctypes.byref(d), *([None] * 7)) == -1
+    assert b"null" in lib.ssmx_last_error()

@@ -32,9 +32,6 @@ def _run(golden, flags, n_out, backend="c"):
 @pytest.mark.parametrize("name", sorted(GOLDEN))
 def test_golden_writers_vs_oracle(gpu, name):
     flags = GOLDEN[name]
-    if not flags["use_residual"]:
-        pytest.skip("non-residual rounding of the reference is not "
-                    "reproduced (see generate_simulator docstring)")
     t_dev, d_dev, t_ref, d_ref = _run(name, flags, 300)
     assert np.array_equal(t_dev, t_ref)
     scale = np.max(np.abs(d_ref))
@@ -45,8 +42,9 @@ def test_golden_writers_vs_oracle(gpu, name):
 
 @pytest.mark.parametrize("name", sorted(GOLDEN))
 def test_golden_csv_prefix(gpu, name):
+    """The writer's own flags, non-residual qubit-couple included: within
+    the 1e-10 parity bar of the golden values over the first 1 us."""
     flags = dict(GOLDEN[name])
-    flags["use_residual"] = True
     coeffs, spins, si, gi, rho = case(name)
     sampler, gens, vmap, kw = prepare_simspins(coeffs, spins, si, gi, **flags)
     _, dens = generate_simulator(sampler, gens, vmap, **kw)(
@@ -55,11 +53,34 @@ def test_golden_csv_prefix(gpu, name):
     ref = load_csv(name)[:mine.size]
     err = np.max(np.abs(mine - ref))
     print(f"{name}: max |golden diff| {err:.3e}")
-    # Residual writers: within the 1e-10 parity bar of the golden values.
-    # qubit-couple was written in non-residual mode, whose reference
-    # rounding (1 + A/4^R loses the diagonal) drifts by ~2e-7 at 1 us;
-    # the residual exponentials here are the exact ones.
-    assert err < (1e-10 if GOLDEN[name]["use_residual"] else 1e-6)
+    assert err < 1e-10
+
+
+#: non-residual variants beyond the golden writer: rotating frame, more
+#: exponentials, damping (which this mode drops), kernel and unitary modes
+NONRESIDUAL = {
+    "qubit-couple-rot": ("qubit-couple", dict(use_rotating=True)),
+    "qubit-couple-e2": ("qubit-couple", dict(number_of_exponentials=2)),
+    "qubit-t1-mK": ("qubit-t1-mK", dict(use_rotating=False)),
+    "qubit-t1-mK-rot-e5": ("qubit-t1-mK", dict(use_rotating=True,
+                                               number_of_exponentials=5)),
+    "qutrit-couple-ker": ("qutrit-couple", dict(use_kernel=True)),
+    "qutrit-couple-unitary": ("qutrit-couple", dict(
+        use_rotating=True, number_of_exponentials=5, use_unitary=True)),
+    "qubit-hyperfine-rot": ("qubit-hyperfine", dict(use_rotating=True)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(NONRESIDUAL))
+def test_nonresidual_variants_vs_oracle(gpu, name):
+    golden, extra = NONRESIDUAL[name]
+    flags = dict(GOLDEN[golden], use_residual=False,
+                 number_of_fine_divisions=20, **extra)
+    t_dev, d_dev, t_ref, d_ref = _run(golden, flags, 100)
+    assert np.array_equal(t_dev, t_ref)
+    err = np.max(np.abs(d_dev - d_ref)) / np.max(np.abs(d_ref))
+    print(f"{name}: rel err {err:.3e}")
+    assert err <= TOL
 
 
 @pytest.mark.parametrize("name", sorted(READERS))
