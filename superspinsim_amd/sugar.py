@@ -218,14 +218,46 @@ def _make_superoperators(ham_quiescent, ham_dependent, jump_quiescent,
     return np.array(dependent), valid_indices, rotating_kw, use_rotating
 
 
+def mesolve_coefficients(hamiltonian_coefficients, jump_coefficients,
+                         use_rotating, intended_coefficients=False):
+    """The sampler ``mesolve`` hands to ``generate_simulator``.
+
+    The reference writes ``sample[k] = coefficient_k(t)`` for the Hamiltonian
+    coefficients followed by ``sqrt`` of the jump coefficients
+    (sugar.py:103-137, 244-257), whatever the frame.  In the rotating frame
+    the generators are exactly the time-dependent ones, so slot k is
+    coefficient k.  Outside it the generator list starts with the quiescent
+    superoperator (:370-374): the reference then multiplies the quiescent
+    part by the first coefficient function and shifts every other one by a
+    slot (the last time-dependent generator gets coefficient 0).  That is
+    reproduced by default; ``intended_coefficients=True`` gives the
+    quiescent generator coefficient 1 and the functions their own slots.
+    """
+    funcs = list(hamiltonian_coefficients) + [
+        (lambda f: (lambda t: np.sqrt(f(t))))(f) for f in jump_coefficients]
+    offset = 1 if (intended_coefficients and not use_rotating) else 0
+
+    def lindbladian(time, sample):
+        if offset:
+            sample[0] = 1.0
+        for k, f in enumerate(funcs):
+            sample[offset + k] = f(time)
+
+    return lindbladian
+
+
 def mesolve(H, rho0, ti, tf, dt, c_ops=None, allowed=None,
             use_rotating=True, use_hermitian=True, number_of_exponentials=5,
-            number_of_fine_divisions=10, use_cuda=True):
+            number_of_fine_divisions=10, use_cuda=True,
+            intended_coefficients=False):
     """QuTiP-like front end (sugar.py:21-161).
 
     Time-dependent jump operators ``[L, f]`` enter with coefficient
     ``sqrt(f(t))`` on the dissipator of L, as in the reference
     (sugar.py:123-137), i.e. the dissipator is scaled by sqrt(f), not f.
+    Outside the rotating frame the coefficients fill the sampler's slots as
+    the reference fills them (see ``mesolve_coefficients``; the keyword
+    ``intended_coefficients`` selects the intended wiring instead).
     """
     from superspinsim_amd.generate_simulator import generate_simulator
     hq, hd, hc = _sort_operators(H, "H")
@@ -233,15 +265,8 @@ def mesolve(H, rho0, ti, tf, dt, c_ops=None, allowed=None,
     lq, ld, lc = _sort_operators(c_ops, "c_ops")
     generators, vmap, rotating_kw, use_rotating = _make_superoperators(
         hq, hd, lq, ld, allowed, use_rotating, use_hermitian)
-
-    funcs = list(hc) + [(lambda f: (lambda t: np.sqrt(f(t))))(f) for f in lc]
-    offset = 0 if use_rotating else 1
-
-    def lindbladian(time, sample):
-        if not use_rotating:
-            sample[0] = 1.0
-        for k, f in enumerate(funcs):
-            sample[offset + k] = f(time)
+    lindbladian = mesolve_coefficients(hc, lc, use_rotating,
+                                       intended_coefficients)
 
     simulator = generate_simulator(
         lindbladian, generators, vmap,

@@ -93,3 +93,18 @@ def test_nv7_generators_are_reference_sized():
     assert set(ops["generators"]) == {"L0", "Gx", "Gy", "Gz", "Gr"}
     assert np.abs(ops["generators"]["Gr"]).max() > 0
 
+
+
+def test_mesolve_sampler_slots_follow_the_reference():
+    """Rotating frame: slot k is coefficient k.  Outside it the reference
+    writes the same slots, so its quiescent generator (slot 0) takes the
+    first coefficient; intended_coefficients=True shifts them by one."""
+    from superspinsim_amd.sugar import mesolve_coefficients
+    f = lambda t: 2.0 + t  # noqa: E731
+    g = lambda t: 9.0  # noqa: E731
+    for rotating, intended, want in (
+            (True, False, [2.5, 3.0, 0.0]), (True, True, [2.5, 3.0, 0.0]),
+            (False, False, [2.5, 3.0, 0.0]), (False, True, [1.0, 2.5, 3.0])):
+        row = np.zeros(3)
+        mesolve_coefficients([f], [g], rotating, intended)(0.5, row)
+        assert row.tolist() == want, (rotating, intended)
