@@ -252,6 +252,37 @@ int ssmx_propagate_queue(const ssmx_propagate_desc *desc,
                          double *out, void *stream);
 
 /*
+ * ssmx_propagate_queue without the counter argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.
+ *
+ * The queue counter lives in a file-scope device array of the library
+ * (256 bytes per device, held until the process ends), cleared by a fill
+ * kernel on `stream` before the launch; nothing is allocated or freed.
+ * Every call on one device shares that counter: calls that overlap on one
+ * device (concurrent streams, threads) race and must be serialised by the
+ * caller.  `stream` must belong to the current device (NULL = its default
+ * stream).  Arguments and results are those of ssmx_propagate_queue;
+ * n_inst + n_waves must not exceed SSMX_NOALLOC_MAX_QUEUE.
+ */
+#define SSMX_NOALLOC_MAX_QUEUE 0x7fffffff
+
+int ssmx_propagate_queue_noalloc(const ssmx_propagate_desc *desc,
+                                 const int32_t *ell_col,
+                                 const double *ell_val,
+                                 const double *cf_weights,
+                                 const double *cf_nodes,
+                                 const ssmx_wave *waves,
+                                 const ssmx_segment *segments,
+                                 const double *rho0, const double *eig,
+                                 const double *obs, const int32_t *order,
+                                 const int64_t *steps,
+                                 const int64_t *read_from, int32_t n_waves,
+                                 int32_t waves_per_block, double *out,
+                                 void *stream);
+
+/*
  * Adjoint (reverse-mode gradient) of the trapezoid-integrated readout
  *   F_o = sum_k w_k obs_o . x_k,  w_0 = w_T = h / 2, other w_k = h,
  * over the same propagation as ssmx_propagate with SSMX_READ_INTEGRAL

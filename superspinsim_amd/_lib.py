@@ -24,6 +24,8 @@ MAX_EXP = 6
 MAX_NODE = 3
 MAX_OBS = 2
 MAX_TAYLOR = 40
+#: n_inst + n_waves limit of ssmx_propagate_queue_noalloc
+NOALLOC_MAX_QUEUE = 0x7fffffff
 ADJ_MAX_TAYLOR = 10
 ADJ_MAX_THETA = 16
 ADJ_MAX_SLOTS = 32
@@ -97,7 +99,7 @@ class Wave(ctypes.Structure):
 
 #: every symbol include/ssmx.h declares
 EXPORTS = ("ssmx_propagate", "ssmx_propagate_dense", "ssmx_propagate_queue",
-           "ssmx_adjoint",
+           "ssmx_propagate_queue_noalloc", "ssmx_adjoint",
            "ssmx_adjoint_workspace_bytes", "ssmx_last_error", "ssmx_version",
            "ssmx_device_count")
 
@@ -131,6 +133,10 @@ def load():
         [ctypes.POINTER(PropagateDesc)] + [vp] * 13 +
         [ctypes.c_int32, ctypes.c_int32, vp, vp])
     lib.ssmx_propagate_queue.restype = ctypes.c_int
+    lib.ssmx_propagate_queue_noalloc.argtypes = (
+        [ctypes.POINTER(PropagateDesc)] + [vp] * 12 +
+        [ctypes.c_int32, ctypes.c_int32, vp, vp])
+    lib.ssmx_propagate_queue_noalloc.restype = ctypes.c_int
     lib.ssmx_propagate_dense.argtypes = [ctypes.POINTER(DenseDesc)] + \
         [vp] * 7
     lib.ssmx_propagate_dense.restype = ctypes.c_int

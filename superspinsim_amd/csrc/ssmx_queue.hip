@@ -16,6 +16,8 @@
 
 #include "ssmx_common.h"
 
+#include <mutex>
+
 namespace {
 
 using ssmx::assemble_row;
@@ -379,6 +381,106 @@ int validate(const ssmx_propagate_desc *d, int n_waves, int waves_per_block)
     return SSMX_OK;
 }
 
+int queue_launch(const ssmx_propagate_desc *desc, const int32_t *ell_col,
+                 const double *ell_val, const double *cf_weights,
+                 const double *cf_nodes, const ssmx_wave *waves,
+                 const ssmx_segment *segments, const double *rho0,
+                 const double *eig, const double *obs, const int32_t *order,
+                 const int64_t *steps, const int64_t *read_from,
+                 int32_t *counter, int32_t n_waves, int32_t waves_per_block,
+                 double *out, hipStream_t s)
+{
+    QParams p;
+    p.d = *desc;
+    p.ell_col = ell_col;
+    p.ell_val = ell_val;
+    p.w = cf_weights;
+    p.nodes = cf_nodes;
+    p.waves = waves;
+    p.segs = segments;
+    p.rho0 = rho0;
+    p.eig = eig;
+    p.obs = obs;
+    p.order = order;
+    p.steps = steps;
+    p.read_from = read_from;
+    p.counter = counter;
+    p.out = out;
+    switch (desc->slots) {
+    case 1: case 2: return launch<2>(p, n_waves, waves_per_block, s);
+    case 3: return launch<3>(p, n_waves, waves_per_block, s);
+    case 4: return launch<4>(p, n_waves, waves_per_block, s);
+    case 5: return launch<5>(p, n_waves, waves_per_block, s);
+    case 6: return launch<6>(p, n_waves, waves_per_block, s);
+    case 7: return launch<7>(p, n_waves, waves_per_block, s);
+    case 8: return launch<8>(p, n_waves, waves_per_block, s);
+    case 9: case 10: return launch<10>(p, n_waves, waves_per_block, s);
+    case 11: case 12: return launch<12>(p, n_waves, waves_per_block, s);
+    default: break;
+    }
+    if (desc->slots <= 16) return launch<16>(p, n_waves, waves_per_block, s);
+    if (desc->slots <= 24) return launch<24>(p, n_waves, waves_per_block, s);
+    if (desc->slots <= 32) return launch<32>(p, n_waves, waves_per_block, s);
+    return launch<64>(p, n_waves, waves_per_block, s);
+}
+
+int check_pointers(const ssmx_propagate_desc *desc, const int32_t *ell_col,
+                   const double *ell_val, const double *cf_weights,
+                   const double *cf_nodes, const ssmx_wave *waves,
+                   const ssmx_segment *segments, const double *rho0,
+                   const double *eig, const double *obs,
+                   const int32_t *order, const int64_t *steps,
+                   const int64_t *read_from, const double *out)
+{
+    if (!ell_col || !ell_val || !cf_weights || !cf_nodes || !waves ||
+        !segments || !rho0 || !obs || !order || !steps || !read_from || !out)
+        return set_error(SSMX_EINVAL, "null required pointer");
+    if ((desc->flags & SSMX_ROTATING) && !eig)
+        return set_error(SSMX_EINVAL, "rotating frame needs eig factors");
+    return SSMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free queue launches.
+//
+// `scratch` holds the work-queue counter of ssmx_propagate_queue_noalloc
+// (element 0; the rest pads the array to 256 bytes).  Every device the
+// library runs on keeps its own copy for the life of the process.  In host
+// code the name `scratch` is only a host-side stand-in: the device address
+// comes from hipGetSymbolAddress, looked up once per device (with that
+// device current) and kept in g_scratch[device].
+// ---------------------------------------------------------------------------
+constexpr int kScratchInts = 64;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) int32_t scratch[kScratchInts];
+
+std::mutex g_scratch_mutex;
+int32_t *g_scratch[kMaxDevices] = {};
+
+__global__ void clear_queue_counter(int32_t *counter)
+{
+    if (threadIdx.x == 0)
+        counter[0] = 0;
+}
+
+// Device address of this device's `scratch` (nullptr and an error message
+// on failure).
+int32_t *scratch_counter(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(SSMX_ELAUNCH, "hipGetSymbolAddress(scratch): %s",
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[device] = static_cast<int32_t *>(ptr);
+    }
+    return g_scratch[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -398,45 +500,83 @@ int ssmx_propagate_queue(const ssmx_propagate_desc *desc,
     int rc = validate(desc, n_waves, waves_per_block);
     if (rc)
         return rc;
-    if (!ell_col || !ell_val || !cf_weights || !cf_nodes || !waves ||
-        !segments || !rho0 || !obs || !order || !steps || !read_from ||
-        !counter || !out)
+    rc = check_pointers(desc, ell_col, ell_val, cf_weights, cf_nodes, waves,
+                        segments, rho0, eig, obs, order, steps, read_from,
+                        out);
+    if (rc)
+        return rc;
+    if (!counter)
         return set_error(SSMX_EINVAL, "null required pointer");
-    if ((desc->flags & SSMX_ROTATING) && !eig)
-        return set_error(SSMX_EINVAL, "rotating frame needs eig factors");
-    QParams p;
-    p.d = *desc;
-    p.ell_col = ell_col;
-    p.ell_val = ell_val;
-    p.w = cf_weights;
-    p.nodes = cf_nodes;
-    p.waves = waves;
-    p.segs = segments;
-    p.rho0 = rho0;
-    p.eig = eig;
-    p.obs = obs;
-    p.order = order;
-    p.steps = steps;
-    p.read_from = read_from;
-    p.counter = counter;
-    p.out = out;
+    return queue_launch(desc, ell_col, ell_val, cf_weights, cf_nodes, waves,
+                        segments, rho0, eig, obs, order, steps, read_from,
+                        counter, n_waves, waves_per_block, out,
+                        static_cast<hipStream_t>(stream));
+}
+
+int ssmx_propagate_queue_noalloc(const ssmx_propagate_desc *desc,
+                                 const int32_t *ell_col,
+                                 const double *ell_val,
+                                 const double *cf_weights,
+                                 const double *cf_nodes,
+                                 const ssmx_wave *waves,
+                                 const ssmx_segment *segments,
+                                 const double *rho0, const double *eig,
+                                 const double *obs, const int32_t *order,
+                                 const int64_t *steps,
+                                 const int64_t *read_from, int32_t n_waves,
+                                 int32_t waves_per_block, double *out,
+                                 void *stream)
+{
+    ssmx::error_buffer()[0] = '\0';
+    int rc = validate(desc, n_waves, waves_per_block);
+    if (rc)
+        return rc;
+    rc = check_pointers(desc, ell_col, ell_val, cf_weights, cf_nodes, waves,
+                        segments, rho0, eig, obs, order, steps, read_from,
+                        out);
+    if (rc)
+        return rc;
+    // The counter ends at n_inst + n_waves; the scratch holds one int32.
+    if (static_cast<int64_t>(desc->n_inst) + n_waves >
+        static_cast<int64_t>(SSMX_NOALLOC_MAX_QUEUE))
+        return set_error(SSMX_EINVAL,
+                         "n_inst + n_waves = %lld exceeds the scratch "
+                         "counter's limit %d",
+                         static_cast<long long>(desc->n_inst) + n_waves,
+                         SSMX_NOALLOC_MAX_QUEUE);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    switch (desc->slots) {
-    case 1: case 2: return launch<2>(p, n_waves, waves_per_block, s);
-    case 3: return launch<3>(p, n_waves, waves_per_block, s);
-    case 4: return launch<4>(p, n_waves, waves_per_block, s);
-    case 5: return launch<5>(p, n_waves, waves_per_block, s);
-    case 6: return launch<6>(p, n_waves, waves_per_block, s);
-    case 7: return launch<7>(p, n_waves, waves_per_block, s);
-    case 8: return launch<8>(p, n_waves, waves_per_block, s);
-    case 9: case 10: return launch<10>(p, n_waves, waves_per_block, s);
-    case 11: case 12: return launch<12>(p, n_waves, waves_per_block, s);
-    default: break;
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess)
+        return set_error(SSMX_EDEVICE, "hipGetDevice: %s",
+                         hipGetErrorString(e));
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(SSMX_EDEVICE, "device %d outside 0..%d", device,
+                         kMaxDevices - 1);
+    if (s) {
+        int stream_device = -1;
+        e = hipStreamGetDevice(s, &stream_device);
+        if (e != hipSuccess)
+            return set_error(SSMX_EINVAL, "hipStreamGetDevice: %s",
+                             hipGetErrorString(e));
+        if (stream_device != device)
+            return set_error(SSMX_EINVAL,
+                             "stream belongs to device %d, current device "
+                             "is %d", stream_device, device);
     }
-    if (desc->slots <= 16) return launch<16>(p, n_waves, waves_per_block, s);
-    if (desc->slots <= 24) return launch<24>(p, n_waves, waves_per_block, s);
-    if (desc->slots <= 32) return launch<32>(p, n_waves, waves_per_block, s);
-    return launch<64>(p, n_waves, waves_per_block, s);
+    int32_t *counter = scratch_counter(device);
+    if (!counter)
+        return SSMX_ELAUNCH;
+    // a fill kernel, not a memset, so that a captured graph replays it
+    hipLaunchKernelGGL(clear_queue_counter, dim3(1), dim3(64), 0, s,
+                       counter);
+    e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(SSMX_ELAUNCH, "counter clear: %s",
+                         hipGetErrorString(e));
+    return queue_launch(desc, ell_col, ell_val, cf_weights, cf_nodes, waves,
+                        segments, rho0, eig, obs, order, steps, read_from,
+                        counter, n_waves, waves_per_block, out, s);
 }
 
 }  // extern "C"

@@ -147,6 +147,28 @@ class ProtocolBatch:
         _lib.check(rc)
         return self.out
 
+    def launch_noalloc(self, stream=None):
+        """Launch through ``ssmx_propagate_queue_noalloc``: the library's
+        own per-device counter, cleared on ``stream``; no allocation on
+        this path.  Overlapping calls on one device race."""
+        if not self.prepared:
+            raise RuntimeError("prepare() first")
+        torch = _torch()
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        desc = self._desc()
+        prop = self.prop
+        rc = _lib.load().ssmx_propagate_queue_noalloc(
+            ctypes.byref(desc), _ptr(prop.d_col), _ptr(prop.d_val),
+            _ptr(prop.d_w), _ptr(prop.d_nodes), _ptr(self.d_waves),
+            _ptr(self.d_segs), _ptr(self.d_rho0), None, _ptr(self.d_obs),
+            _ptr(self.d_order), _ptr(self.d_steps), _ptr(self.d_read_from),
+            ctypes.c_int32(self.n_waves),
+            ctypes.c_int32(self.waves_per_block), _ptr(self.out),
+            ctypes.c_void_p(stream))
+        _lib.check(rc)
+        return self.out
+
     def run(self):
         """Eager launch; returns the (n_inst, n_obs) readouts (device)."""
         return self.launch()
