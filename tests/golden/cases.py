@@ -16,22 +16,18 @@ import os
 
 import numpy as np
 
+from superspinsim_amd.golden import density_metric as reference_metric
+from superspinsim_amd.golden import flatten_density, load_density
+
 CSV_DIR = os.path.join(os.path.dirname(__file__), "csv")
 
 
 def load_csv(name):
-    return np.loadtxt(os.path.join(CSV_DIR, f"{name}.csv"), delimiter=",")
+    return load_density(name, CSV_DIR)
 
 
-def flatten_density(density):
-    """Reference golden format: every 29th (re, im) entry, C order."""
-    expanded = np.stack([density.real, density.imag], axis=-1)
-    return expanded.ravel()[::29]
-
-
-def reference_metric(this, reference):
-    """``||this - ref||_2 / size`` as in ``test/tests/util.py:42-59``."""
-    return float(np.sqrt(np.sum((this - reference) ** 2)) / this.size)
+__all__ = ["CSV_DIR", "GOLDEN", "READERS", "case", "flatten_density",
+           "load_csv", "reference_metric"]
 
 
 def _fields(frequency, amplitude, cutoff=None):
