@@ -214,6 +214,44 @@ int ssmx_propagate_dense(const ssmx_dense_desc *desc, const double *gens,
                          void *stream);
 
 /*
+ * Chain the per-chunk propagators of a time-chunked run.
+ *
+ * A run of N outputs is cut into n_chunks chunks of chunk_out outputs (the
+ * last may be shorter).  ops holds, for every instance, chunk p and basis
+ * column c, the full-state history of ssmx_propagate started from the
+ * unit vector e_c over chunk p (time-shifted to the chunk's start): the
+ * propagator U_p(t_k) column by column.  With x_0 = rho0 the kernel writes
+ * the state U_p(t_k) x_p at every global output p chunk_out + k (or its
+ * n_obs readout rows) and carries x_{p+1} = U_p(t_last) x_p to the next
+ * chunk.  This is the reference's serial prefix product and application
+ * to rho0 (generate_simulator.py:1204-1257, 1308-1321, 1325-1390) over
+ * chunks instead of single outputs.
+ *
+ * Member (inst, p, c) = (inst * n_chunks + p) * n + c sits in system
+ * member / pack at block member % pack of a block-diagonal ssmx_propagate
+ * run of pack * n rows:
+ *   ops   double [systems][chunk_out][pack * n], systems =
+ *         ceil(n_inst * n_chunks * n / pack), standard basis
+ *   rho0  double [n_inst][n]
+ *   obs   double [n_obs][n] (n_obs 1..2), or NULL with n_obs = 0
+ *   out   n_obs = 0: double [n_inst][n_out][n]; else [n_inst][n_out][n_obs]
+ */
+typedef struct ssmx_chunk_desc {
+    int32_t n;          /* state dimension, 1..64 */
+    int32_t n_inst;
+    int32_t n_chunks;   /* P */
+    int32_t chunk_out;  /* L outputs per chunk */
+    int32_t n_out;      /* N, (P - 1) L < N <= P L */
+    int32_t pack;       /* members per system, pack * n <= 64 */
+    int32_t n_obs;      /* 0 (full states) or 1..2 readout rows */
+    int32_t reserved;   /* 0 */
+} ssmx_chunk_desc;
+
+int ssmx_chunk_combine(const ssmx_chunk_desc *desc, const double *ops,
+                       const double *rho0, const double *obs, double *out,
+                       void *stream);
+
+/*
  * Persistent work-queue propagation of protocols with uneven lengths.
  *
  * Instance i runs steps[i] propagator steps (node times as ssmx_propagate

@@ -83,6 +83,13 @@ class DenseDesc(ctypes.Structure):
     ]
 
 
+class ChunkDesc(ctypes.Structure):
+    """Mirror of ``ssmx_chunk_desc`` (include/ssmx.h)."""
+    _fields_ = [(f, ctypes.c_int32) for f in (
+        "n", "n_inst", "n_chunks", "chunk_out", "n_out", "pack", "n_obs",
+        "reserved")]
+
+
 class Segment(ctypes.Structure):
     """Mirror of ``ssmx_segment``."""
     _fields_ = [("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
@@ -98,7 +105,8 @@ class Wave(ctypes.Structure):
 
 
 #: every symbol include/ssmx.h declares
-EXPORTS = ("ssmx_propagate", "ssmx_propagate_dense", "ssmx_propagate_queue",
+EXPORTS = ("ssmx_propagate", "ssmx_propagate_dense", "ssmx_chunk_combine",
+           "ssmx_propagate_queue",
            "ssmx_propagate_queue_noalloc", "ssmx_adjoint",
            "ssmx_adjoint_workspace_bytes", "ssmx_last_error", "ssmx_version",
            "ssmx_device_count")
@@ -140,6 +148,8 @@ def load():
     lib.ssmx_propagate_dense.argtypes = [ctypes.POINTER(DenseDesc)] + \
         [vp] * 7
     lib.ssmx_propagate_dense.restype = ctypes.c_int
+    lib.ssmx_chunk_combine.argtypes = [ctypes.POINTER(ChunkDesc)] + [vp] * 5
+    lib.ssmx_chunk_combine.restype = ctypes.c_int
     lib.ssmx_adjoint.argtypes = [ctypes.POINTER(AdjointDesc)] + [vp] * 17
     lib.ssmx_adjoint.restype = ctypes.c_int
     lib.ssmx_adjoint_workspace_bytes.argtypes = [

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(LIB_DIR, "_obj")
 SOURCES = ["ssmx_propagate.hip", "ssmx_adjoint.hip", "ssmx_queue.hip",
-           "ssmx_wide.hip", "ssmx_adjoint_wide.hip", "ssmx_dense.hip"]
+           "ssmx_wide.hip", "ssmx_adjoint_wide.hip", "ssmx_dense.hip",
+           "ssmx_chunks.hip"]
 
 # -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32 (see DESIGN notes on
 # packed FP32 on this hardware); the host pass warns that it does not know

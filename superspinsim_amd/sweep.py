@@ -16,13 +16,14 @@ reference's examples use), piecewise-constant amplitude tables
 (``omega = 0``) and gated optical excitation.
 """
 
+import ctypes
 import math
 
 import numpy as np
 
 from superspinsim_amd import _lib, quadratures
 from superspinsim_amd.propagator import (
-    Propagator, coefficient_samples, device_tensor, eig_lane_factors,
+    Propagator, _ptr, coefficient_samples, device_tensor, eig_lane_factors,
     rotating_tables)
 
 _f64 = np.float64
@@ -86,6 +87,19 @@ class Waveform:
             # segments of one generator do not overlap
             if segs:
                 out[g] += max(abs(s[2]) for s in segs)
+        return out
+
+    def shifted(self, offset):
+        """The waveform seen from a clock started at ``offset``:
+        ``shifted(T)(t) = self(t + T)`` up to rounding.  Segment windows move
+        by -T and each tone's phase gains omega T (reduced mod 2 pi)."""
+        out = Waveform(self.n_gen, self.base)
+        for g, segs in enumerate(self.segments):
+            for (ts, te, amp, om, ph), tag in zip(segs, self.params[g]):
+                out.segments[g].append(
+                    (ts - offset, te - offset, amp, om,
+                     math.remainder(ph + om * offset, math.tau)))
+                out.params[g].append(tag)
         return out
 
     def sampler(self, rotating_offset=0):
@@ -418,6 +432,96 @@ class BatchSimulator:
         self._keep = run
         return run.result()
 
+    # ------------------------------------------------------------------
+    # time-chunked runs (small systems, few instances)
+
+    def chunk_plan(self, n_inst, N, resident=None):
+        """(chunks P, outputs per chunk L, pack k) of a chunked run.
+
+        Every instance's N outputs are cut into P chunks of L; each chunk
+        propagates the n basis vectors, k per wavefront.  P minimises the
+        launch's length in wavefront rounds times chunk length,
+        ceil(waves / resident) L, smallest P first.
+        """
+        if self.rotating or self.n > 32:
+            raise ValueError("chunked runs need the lab frame and n <= 32")
+        k = max(1, min(_lib.MAX_N // self.n, _lib.MAX_GEN - 1))
+        if resident is None:
+            torch = __import__("torch")
+            resident = 16 * torch.cuda.get_device_properties(
+                0).multi_processor_count
+        best = None
+        for P in range(1, N + 1):
+            L = -(-N // P)
+            if (P - 1) * L >= N:        # the same split as a smaller P
+                continue
+            waves = -(-(n_inst * P * self.n) // k)
+            cost = -(-waves // resident) * L
+            if best is None or cost < best[0]:
+                best = (cost, P, L)
+        return best[1], best[2], k
+
+    def prepare_chunked(self, waves, rho0, time_start, time_end, time_step,
+                        obs=None, read="obs", chunks=None):
+        """``prepare`` for a run cut into time chunks (``ChunkedRun``).
+
+        Each instance's outputs are split into P chunks; every chunk
+        propagates the n basis vectors from the chunk's start (waveforms
+        time-shifted to it, ``Waveform.shifted``), all chunks of all
+        instances in one launch, and ``ssmx_chunk_combine`` chains the chunk
+        propagators from rho0.  Same readouts as ``run`` ("obs" or "full");
+        the node times of a chunk are evaluated from the chunk's start, so
+        results equal ``run``'s to rounding, not bit for bit.
+        """
+        if read not in ("obs", "full"):
+            raise ValueError("chunked runs read out 'obs' or 'full'")
+        N = int((time_end - time_start) / time_step)
+        n_inst = len(waves)
+        if chunks is None:
+            P, L, k = self.chunk_plan(n_inst, N)
+        else:
+            P = max(1, min(int(chunks), N))
+            L = -(-N // P)
+            P = -(-N // L)
+            k = max(1, min(_lib.MAX_N // self.n, _lib.MAX_GEN - 1))
+        n = self.n
+        members = []
+        for w in waves:
+            for p in range(P):
+                shifted = w.shifted(time_start + p * L * time_step)
+                members.extend([shifted] * n)
+        basis = np.tile(np.eye(n), (n_inst * P, 1))
+        saved = self.pack
+        self.pack = k
+        try:
+            run = self.prepare(members, basis, 0.0,
+                               L * time_step * (1 + 1e-12), time_step,
+                               read="full")
+        finally:
+            self.pack = saved
+        rho0 = np.atleast_2d(np.asarray(rho0, dtype=_f64))
+        if rho0.shape[0] == 1 and n_inst > 1:
+            rho0 = np.repeat(rho0, n_inst, axis=0)
+        obs_rows = None
+        if read == "obs":
+            obs_rows = np.atleast_2d(np.asarray(obs, dtype=_f64))
+        self.last_plan = dict(self.last_plan, n_out=N,
+                              steps=N * self.F, chunks=P, chunk_out=L,
+                              chunk_pack=run.k)
+        return ChunkedRun(run, n_inst, N, P, L, device_tensor(rho0),
+                          None if obs_rows is None else
+                          device_tensor(obs_rows))
+
+    def run_chunked(self, waves, rho0, time_start, time_end, time_step,
+                    obs=None, read="obs", chunks=None):
+        """``run`` by time chunks (see ``prepare_chunked``)."""
+        run = self.prepare_chunked(waves, rho0, time_start, time_end,
+                                   time_step, obs=obs, read=read,
+                                   chunks=chunks)
+        run.launch()
+        self._keep = run
+        return run.result()
+
     def step_flops(self, plan=None):
         """Algorithmic FP64 FLOPs of one propagator step of one instance.
 
@@ -432,6 +536,42 @@ class BatchSimulator:
         per_exp = 2 * K * nnz + n + p["substeps"] * p["taylor"] * (
             2 * nnz + 2 * n)
         return J * per_exp + (4 * n if self.rotating else 0)
+
+
+class ChunkedRun:
+    """Device inputs of a time-chunked run (``prepare_chunked``): ``launch``
+    propagates every chunk's basis vectors, then chains the chunks."""
+
+    def __init__(self, chunk_run, n_inst, N, P, L, d_rho0, d_obs):
+        torch = __import__("torch")
+        self.chunk_run, self.n_inst, self.N = chunk_run, n_inst, N
+        self.P, self.L = P, L
+        self.d_rho0, self.d_obs = d_rho0, d_obs
+        n = chunk_run.sim.n
+        n_obs = 0 if d_obs is None else d_obs.shape[0]
+        self.desc = _lib.ChunkDesc(n=n, n_inst=n_inst, n_chunks=P,
+                                   chunk_out=L, n_out=N, pack=chunk_run.k,
+                                   n_obs=n_obs, reserved=0)
+        shape = (n_inst, N, n_obs if n_obs else n)
+        self.out = torch.empty(shape, dtype=torch.float64, device="cuda")
+
+    def launch(self):
+        torch = __import__("torch")
+        ops = self.chunk_run.launch()
+        systems = -(-self.n_inst * self.P * self.desc.n // self.desc.pack)
+        want = (systems, self.L, self.desc.pack * self.desc.n)
+        if tuple(ops.shape) != want or not ops.is_contiguous():
+            raise RuntimeError(f"chunk propagators {tuple(ops.shape)}, "
+                               f"expected contiguous {want}")
+        stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.load().ssmx_chunk_combine(
+            ctypes.byref(self.desc), _ptr(ops), _ptr(self.d_rho0),
+            _ptr(self.d_obs), _ptr(self.out), ctypes.c_void_p(stream)))
+        self._keep = ops
+        return self.out
+
+    def result(self):
+        return self.out
 
 
 class PreparedRun:

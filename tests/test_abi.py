@@ -220,3 +220,28 @@ def test_dense_null_pointers_are_rejected(lib):
     d = _ddesc()
     assert lib.ssmx_propagate_dense(ctypes.byref(d), *([None] * 7)) == -1
     assert b"null" in lib.ssmx_last_error()
+
+
+def _cdesc(**kw):
+    base = dict(n=9, n_inst=2, n_chunks=3, chunk_out=4, n_out=10, pack=7,
+                n_obs=1, reserved=0)
+    base.update(kw)
+    return _lib.ChunkDesc(**base)
+
+
+@pytest.mark.parametrize("bad", [
+    dict(n=0), dict(n=65), dict(n_inst=0), dict(n_chunks=0),
+    dict(chunk_out=0), dict(n_out=13), dict(n_out=8), dict(pack=0),
+    dict(pack=8), dict(n_obs=3), dict(n_obs=-1), dict(reserved=1),
+])
+def test_invalid_chunk_descriptor_is_rejected(lib, bad):
+    d = _cdesc(**bad)
+    dummy = ctypes.c_void_p(8)
+    assert lib.ssmx_chunk_combine(ctypes.byref(d), *([dummy] * 5)) == -1
+    assert lib.ssmx_last_error().decode()
+
+
+def test_chunk_null_pointers_are_rejected(lib):
+    d = _cdesc()
+    assert lib.ssmx_chunk_combine(ctypes.byref(d), *([None] * 5)) == -1
+    assert b"null" in lib.ssmx_last_error()

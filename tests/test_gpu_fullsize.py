@@ -67,6 +67,31 @@ def test_full_length_sweep_point_vs_oracle(gpu, factory, freq):
     assert np.max(np.abs(ref)) > 1e-3
 
 
+def test_full_length_chunked_3_level_vs_oracle(gpu):
+    """The 3-level sweep by time chunks (its benchmark path): one sweep
+    point at full length against the oracle, and the other points of a
+    16-point batch against the sequential run."""
+    wl = configs.odmr3(16)
+    wl.frequencies = np.linspace(2.77e9, 2.97e9, 16)
+    waves = wl.waveforms()
+    sim = wl.simulator()
+    args = (waves, wl.rho0, wl.time_start, wl.time_end, wl.time_step)
+    got = sim.run_chunked(*args, obs=wl.obs, read="obs")[:, :, 0]
+    got = got.cpu().numpy()
+    plan = sim.last_plan
+    print(f"{wl.name}: 16 x {wl.steps_per_instance()} steps in "
+          f"{plan['chunks']} chunks of {plan['chunk_out']} outputs",
+          flush=True)
+    seq = sim.run(*args, obs=wl.obs, read="obs")[:, :, 0].cpu().numpy()
+    diff = np.max(np.abs(got - seq)) / np.max(np.abs(seq))
+    print(f"  chunked vs sequential rel diff {diff:.3e}", flush=True)
+    assert diff <= 1e-11
+    ref = _oracle_full_trace(wl, waves[7])
+    err = np.max(np.abs(got[7] - ref)) / np.max(np.abs(ref))
+    print(f"  chunked vs oracle rel err {err:.3e}", flush=True)
+    assert err <= TOL
+
+
 def test_odmr7_full_length_vectorisations_agree(gpu):
     freqs = np.array([2.73e9, 2.78e9, 2.87e9, 2.96e9])
     traces = []

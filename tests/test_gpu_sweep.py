@@ -78,3 +78,42 @@ def test_sweep_integral_is_trapezoid(gpu):
     full = np.concatenate([np.full((2, 1), f0), trace], axis=1)
     ref = np.sum(0.5 * h * (full[:, 1:] + full[:, :-1]), axis=1)
     assert np.max(np.abs(integ - ref) / np.abs(ref)) < 1e-13
+
+
+@pytest.mark.parametrize("chunks", [None, 1, 5, 40])
+def test_chunked_sweep_vs_sequential_and_oracle(gpu, chunks):
+    """Time-chunked runs (chunk propagators chained by
+    ssmx_chunk_combine) against the sequential run and the oracle."""
+    wl = configs.odmr3(3)
+    wl.frequencies = np.linspace(2.77e9, 2.97e9, 3)
+    waves = wl.waveforms()
+    sim = wl.simulator()
+    n_out = 40
+    t1 = n_out * wl.time_step * (1 + 1e-12)
+    print(f"{wl.name}: 3 x {n_out} outputs, chunks={chunks}", flush=True)
+    seq = sim.run(waves, wl.rho0, 0.0, t1, wl.time_step, obs=wl.obs,
+                  read="obs")[:, :, 0].cpu().numpy()
+    got = sim.run_chunked(waves, wl.rho0, 0.0, t1, wl.time_step, obs=wl.obs,
+                          read="obs", chunks=chunks)[:, :, 0].cpu().numpy()
+    print(f"  plan {sim.last_plan['chunks']} x {sim.last_plan['chunk_out']}"
+          f", pack {sim.last_plan['chunk_pack']}", flush=True)
+    assert got.shape == seq.shape == (3, n_out)
+    assert np.max(np.abs(got - seq)) / np.max(np.abs(seq)) <= 1e-11
+    for k, w in enumerate(waves):
+        ref = _oracle_trace(wl, w, n_out)
+        err = np.max(np.abs(got[k] - ref)) / np.max(np.abs(ref))
+        print(f"  f={wl.frequencies[k]:.4e}: rel err {err:.3e}")
+        assert err <= TOL
+
+
+def test_chunked_full_states_match_sequential(gpu):
+    wl = configs.odmr3(2)
+    sim = wl.simulator()
+    waves = wl.waveforms()
+    t1 = 23 * wl.time_step * (1 + 1e-12)
+    seq = sim.run(waves, wl.rho0, 0.0, t1, wl.time_step,
+                  read="full").cpu().numpy()
+    got = sim.run_chunked(waves, wl.rho0, 0.0, t1, wl.time_step,
+                          read="full", chunks=4).cpu().numpy()
+    assert got.shape == seq.shape
+    assert np.max(np.abs(got - seq)) / np.max(np.abs(seq)) <= 1e-11

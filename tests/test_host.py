@@ -482,3 +482,29 @@ def test_trapezoid_window():
     want = 0.5 * 0.5 * sum(s[0, k] + s[0, k + 1] for k in range(2, 7))
     assert got[0] == want
     assert trapezoid_window(s, [5], [5], 0.5)[0] == 0.0
+
+
+def test_waveform_shift_reproduces_the_later_clock():
+    """shifted(T)(t) = w(t + T) to rounding: tones keep their phase, windows
+    move."""
+    import math
+    from superspinsim_amd.sweep import Waveform
+    w = Waveform(3, base=[1.0, 0.0, 0.2])
+    w.tone(1, 3e-4, 2.87e9, phase=0.4)
+    w.add(2, 1e-6, 2e-6, 0.5, omega=1e7)
+    T = 7.3e-7
+    sw = w.shifted(T)
+    for t in np.linspace(0, 2e-6, 97):
+        a, b = sw(t), w(t + T)
+        assert np.allclose(a, b, rtol=0, atol=1e-14 * 0.5), t
+    assert sw.segments[2][0][0] == 1e-6 - T
+    assert abs(sw.segments[1][0][4]) <= math.pi
+
+
+def test_chunk_plan_covers_every_output():
+    from superspinsim_amd.sweep import BatchSimulator
+    sim = BatchSimulator(np.zeros((2, 9, 9)), 2, 50)
+    for n_inst, N in ((256, 999), (3, 40), (1, 7), (5, 1)):
+        P, L, k = sim.chunk_plan(n_inst, N, resident=4096)
+        assert (P - 1) * L < N <= P * L
+        assert k * 9 <= 64
