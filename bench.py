@@ -11,10 +11,11 @@ timed region; ``value_pcie_inclusive`` repeats the steps with the
 per-instance inputs uploaded from pinned host memory before each launch.
 
 The gradient leg runs the 1024-point version of the sweep with a
-200-segment microwave amplitude: forward with checkpoints every 1000 steps,
-then the adjoint (recomputation + reverse sweep) for the squared error
-against targets simulated at parameters perturbed by up to 2 %; the ratio
-is (forward with checkpoints + backward) / plain forward, wall time.
+200-segment microwave amplitude: a forward that keeps every exponential's
+input in HBM (105 GB), then the adjoint (forward terms from the stored
+inputs + reverse sweep) for the squared error against targets simulated at
+parameters perturbed by up to 2 %; the ratio is (forward with stored inputs
++ backward) / plain forward, wall time.
 
 The protocol leg runs 16384 pulsed Ramsey / Hahn-echo protocols of 58k-742k
 steps in one persistent work-queue launch (autotuned shape, replayed from a
@@ -218,6 +219,10 @@ def adjoint_leg(args):
         "adjoint_degree": int(state["taylor_adj"]),
         "ell_slots": int(state["prop"].ell.slots),
         "ell_slots_adjoint": int(state["slots_t"]),
+        "setup_s": setup_s,
+        "stored_inputs": bool(st.get("stored")),
+        "stored_input_bytes": pb.stored_input_bytes(state["n_inst"])
+        if st.get("stored") else 0,
         "forward_s": med["forward"],
         "forward_ckpt_s": med["forward_ckpt"],
         "backward_s": med["backward"],

@@ -73,6 +73,8 @@ extern "C" {
 #define SSMX_COEF_WAVE 2     /* coefficients from waveform descriptors */
 #define SSMX_READ_FULL 4     /* readout: whole state vector (n values) */
 #define SSMX_READ_INTEGRAL 8 /* readout: trapezoid integral of obs rows */
+#define SSMX_STORE_INPUTS 16 /* state_out: every exponential's input, for
+                                ssmx_adjoint (see ssmx_propagate) */
 
 /* limits */
 #define SSMX_MAX_N 64       /* one wavefront per instance */
@@ -150,7 +152,12 @@ typedef struct ssmx_wave {
  *            READ_INTEGRAL: double [n_inst * n_obs]
  * state_out  ckpt_every = 0: double [n_inst * n] final state (or NULL);
  *            ckpt_every = K: double [n_inst * ceil(n_out * fine / K) * n],
- *            the state before every K-th step (input of ssmx_adjoint)
+ *            the state before every K-th step (input of ssmx_adjoint);
+ *            SSMX_STORE_INPUTS: double [n_inst][n_out * fine][n_exp][64],
+ *            the input of every exponential (rows >= n hold 0), for
+ *            ssmx_adjoint's stored-input mode; needs n <= 64, the lab
+ *            frame, substeps 1, ckpt_every 0, taylor 11, slots <= 8 and at
+ *            least two matrices
  *
  * Replaces: generate_simulator.py:1945-2116 (simulate's kernel sequence).
  */

@@ -16,6 +16,7 @@ SSMX_ROTATING = 1
 SSMX_COEF_WAVE = 2
 SSMX_READ_FULL = 4
 SSMX_READ_INTEGRAL = 8
+SSMX_STORE_INPUTS = 16
 
 MAX_N = 64
 MAX_N_WIDE = 512

@@ -155,8 +155,8 @@ class GradientProblem:
     def __init__(self, generators, waveforms, rho0, obs, time_start,
                  time_end, time_step, number_of_fine_divisions=1,
                  number_of_exponentials=2, checkpoint_every=1000,
-                 tolerance=1e-16, adjoint_tolerance=1e-13, fd_rel=1e-6,
-                 theta_scale=None):
+                 tolerance=1e-16, adjoint_tolerance=1e-12, fd_rel=1e-6,
+                 theta_scale=None, store_inputs="auto"):
         self.generators = generators
         self.waveforms = waveforms
         self.rho0 = np.asarray(rho0, dtype=_f64)
@@ -177,7 +177,45 @@ class GradientProblem:
         self.theta_scale = None if theta_scale is None else \
             np.asarray(theta_scale, dtype=_f64)
         self.n_out = int((self.t1 - self.t0) / self.dt)
+        if store_inputs not in (True, False, "auto"):
+            raise ValueError("store_inputs is True, False or 'auto'")
+        self.store_inputs = store_inputs
         self._cache = {}
+
+    # ------------------------------------------------------------------
+    def stored_input_bytes(self, n_inst):
+        """Device bytes of every exponential's input (stride 64 rows)."""
+        return n_inst * self.n_out * self.F * self.J * _lib.MAX_N * 8
+
+    def _use_stored_inputs(self, state):
+        """Whether ``forward`` keeps every exponential's input, so that
+        ``backward`` needs no recomputation: supported shapes (n <= 64,
+        degree 11, at most 7 forward and 10 adjoint slots, 2 or 3 matrices)
+        and, with ``store_inputs="auto"``, room for them in 80 % of the free
+        device memory."""
+        prop = state["prop"]
+        if not state.get("derivatives", True):
+            return False                 # a plain forward: no backward
+        ok = (prop.n <= _lib.MAX_N and int(state["taylor"]) == 11 and
+              state["slots"] <= 7 and state["slots_t"] <= 10 and
+              prop.K in (2, 3))
+        if self.store_inputs is False:
+            return False
+        if self.store_inputs is True:
+            if not ok:
+                raise NotImplementedError(
+                    "stored inputs need n <= 64, Taylor degree 11, at most "
+                    "7 / 10 slots and 2 or 3 matrices")
+            return True
+        if not ok:
+            return False
+        torch = _torch()
+        need = self.stored_input_bytes(state["n_inst"])
+        have = state.get("inputs")
+        free, _ = torch.cuda.mem_get_info()
+        if have is not None:
+            free += have.numel() * 8
+        return need <= 0.8 * free
 
     # ------------------------------------------------------------------
     def _fd_steps(self, theta):
@@ -294,6 +332,7 @@ class GradientProblem:
                 slots=prop.ell.slots, slots_t=col.shape[0], slots_f=0)
         state = dict(
             theta=theta, amplitudes=amplitudes, prop=prop, program=program,
+            derivatives=bool(derivatives),
             taylor=m, taylor_adj=ma, theta_gen=theta_gen, grad_gen=grad_gen,
             n_theta=dconst.shape[0] if theta_gen >= 0 else 0,
             d_waves=torch.from_numpy(np.frombuffer(
@@ -317,16 +356,31 @@ class GradientProblem:
             rho0 = np.repeat(rho0, n_inst, axis=0)
         total = self.n_out * self.F
         P = (total + self.K - 1) // self.K
-        ckpt = torch.empty((n_inst, P, prop.n), dtype=torch.float64,
-                           device="cuda")
+        stored = self._use_stored_inputs(state)
+        if stored:
+            # every exponential's input (the backward's forward terms start
+            # from them); the buffer is reused across calls
+            shape = (n_inst, total, prop.J, _lib.MAX_N)
+            ckpt = state.get("inputs")
+            if ckpt is None or tuple(ckpt.shape) != shape:
+                state.pop("inputs", None)
+                state.pop("ckpt", None)
+                ckpt = torch.empty(shape, dtype=torch.float64, device="cuda")
+            state["inputs"] = ckpt
+        else:
+            state.pop("inputs", None)
+            ckpt = torch.empty((n_inst, P, prop.n), dtype=torch.float64,
+                               device="cuda")
         out = prop.run(n_inst=n_inst, fine=self.F, n_out=self.n_out,
                        h=self.dt / self.F, t0=self.t0, dt=self.dt,
                        rho0=device_tensor(rho0), waves=state["d_waves"],
                        segments=state["d_segs"], taylor=state["taylor"],
                        substeps=1, read="integral", obs=self.obs,
-                       state_out=ckpt, ckpt_every=self.K)
+                       state_out=ckpt, ckpt_every=0 if stored else self.K,
+                       store_inputs=stored)
         state["ckpt"] = ckpt
         state["n_ckpt"] = P
+        state["stored"] = stored
         return out, state
 
     def adjoint_desc(self, state):
@@ -336,7 +390,10 @@ class GradientProblem:
             slots=state["slots"], slots_t=state["slots_t"], n_exp=prop.J,
             n_node=prop.S, fine=self.F, n_out=self.n_out,
             taylor=int(state["taylor"]), taylor_adj=int(state["taylor_adj"]),
-            n_obs=self.obs.shape[0], flags=_lib.SSMX_COEF_WAVE, first_out=0,
+            n_obs=self.obs.shape[0],
+            flags=_lib.SSMX_COEF_WAVE | (_lib.SSMX_STORE_INPUTS
+                                         if state.get("stored") else 0),
+            first_out=0,
             partner=1, ckpt_every=self.K, n_theta=int(state["n_theta"]),
             theta_gen=int(state["theta_gen"]),
             grad_gen=int(state["grad_gen"]), reserved=int(state["slots_f"]),

@@ -292,7 +292,8 @@ class GradientWorkload:
 def odmr7_gradient(count=1024, segments=200, seed=3, time_end=2e-6,
                    time_step=2e-9, fine=100, checkpoint_every=1000,
                    use_incoherent=False, tolerance=1e-16,
-                   adjoint_tolerance=1e-13, frequencies=None):
+                   adjoint_tolerance=1e-12, frequencies=None,
+                   store_inputs="auto"):
     """C1-style ODMR sweep with segment amplitudes, as a GradientProblem.
 
     ``frequencies`` overrides the 2.77-2.97 GHz sweep (``count`` points).
@@ -322,7 +323,8 @@ def odmr7_gradient(count=1024, segments=200, seed=3, time_end=2e-6,
     problem = GradientProblem(
         generators, waveforms, base.rho0, base.obs, 0.0, time_end,
         time_step, fine, 2, checkpoint_every, tolerance=tolerance,
-        adjoint_tolerance=adjoint_tolerance, theta_scale=THETA_SCALE)
+        adjoint_tolerance=adjoint_tolerance, theta_scale=THETA_SCALE,
+        store_inputs=store_inputs)
     name = "odmr7-gradient-n49" if not use_incoherent else \
         "odmr7-gradient-n19"
     return GradientWorkload(name, problem, nv7_theta0(), amplitudes, freqs,

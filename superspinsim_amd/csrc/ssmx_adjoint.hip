@@ -34,6 +34,8 @@
 
 #include "ssmx_common.h"
 
+#include <cstdlib>
+
 namespace {
 
 using ssmx::kUnrolledTaylor;
@@ -52,7 +54,9 @@ constexpr int kMaxSlots = 32;
 // Forward-term handoff ring between the two wavefronts of an instance.
 constexpr int kRing = 3;
 // Bound on the polls of one handoff wait; on overrun the kernel finishes
-// and reports NaN gradients instead of waiting on.
+// and reports NaN gradients instead of waiting on.  The environment
+// variable SSMX_ADJOINT_MAX_POLLS lowers it (tests force the overrun path
+// from the host so that every kernel still exits normally).
 constexpr long kMaxPolls = 10000000;
 
 // 1/q, q = 0..40 (entry 0 unused); the recompute uses the same table and
@@ -85,6 +89,7 @@ struct AdjParams {
     double *grad_theta;
     double *grad_seg;
     double *lambda_out;
+    long max_polls;  // bound on one handoff wait
 };
 
 // Scalar factor of matrix m in exponential j of global step `step`
@@ -231,14 +236,17 @@ __device__ __forceinline__ void assemble_reg(double (&a)[SL],
 // MG = 0: generator values staged in LDS (any n_gen); MG = n_gen (1..3):
 // each wavefront keeps its generator values in registers, which removes
 // the LDS reads of every assembly.
-template <int SF, int ST, int MG>
+// MA: adjoint degree (<= kMaxAdj).  STORED: wavefront 0 reads every
+// exponential's input from `ckpt` (a forward run with SSMX_STORE_INPUTS)
+// and runs only the forward terms, no recomputation.
+template <int SF, int ST, int MG, int MA = kMaxAdj, bool STORED = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(2 * kWave, 2 * kWave),
                           amdgpu_waves_per_eu(ST <= 16 ? 2 : 1))) void
 adjoint_kernel(AdjParams p)
 {
+    static_assert(MA >= 1 && MA <= kMaxAdj, "adjoint degree");
     constexpr int SFP = SF + (SF & 1);
     constexpr int STP = ST + (ST & 1);
-    constexpr int MA = kMaxAdj;
     extern __shared__ double lds[];
     const ssmx_adjoint_desc &d = p.d;
     const int M = d.n_gen;
@@ -253,9 +261,13 @@ adjoint_kernel(AdjParams p)
     int *produced =
         reinterpret_cast<int *>(lt + (MG == 0 ? M * STP * kWave : 0));
     int *consumed = produced + 1;
+    // set by either wavefront when one of its handoff waits overran: the
+    // ring may then hold values the other side never saw or overwrote
+    int *overrun = produced + 2;
     if (threadIdx.x == 0) {
         *produced = 0;
         *consumed = 0;
+        *overrun = 0;
     }
 
     if constexpr (MG == 0) {
@@ -330,6 +342,121 @@ adjoint_kernel(AdjParams p)
         long handed = 0;                   // exponentials handed over
         int slot = 0;                      // ring slot of the next one
         bool broken = false;
+
+        if constexpr (STORED) {
+            // Forward terms only: the input of every exponential comes
+            // from the forward run (ckpt = [inst][step][j][64]).  Same
+            // phases as the adjoint wavefront (one barrier each).
+            (void)inv_lane;
+            (void)m_fwd;
+            (void)cur_r;
+            const double *yin = p.ckpt + (size_t)inst * total * J * kWave;
+            for (int q = 0; q <= P; ++q) {
+                if (q >= 1) {
+                    const int vseg = P - q;
+                    const long vk0 = (long)vseg * K;
+                    const long vk1 = vk0 + K < total ? vk0 + K : total;
+                    const long EV = (vk1 - vk0) * J;
+                    long vk = vk1 - 1;     // reverse order: (vk, 0), (vk, 1)
+                    int vj = 0;
+                    long vkb = vk + B;     // batch top (forces a load)
+                    double vfac = 0.0;
+                    // inputs prefetched two exponentials ahead (the loads
+                    // stream from HBM; positions before the segment's first
+                    // step are clamped to it, their values unused)
+                    auto next = [&](long &k, int &j) {
+                        if (++j == J) {
+                            j = 0;
+                            --k;
+                        }
+                        if (k < vk0) {
+                            k = vk0;
+                            j = J - 1;
+                        }
+                    };
+                    long pk = vk;
+                    int pj = vj;
+                    double ypre = yin[((size_t)pk * J + pj) * kWave + lane];
+                    next(pk, pj);
+                    double ypre2 = yin[((size_t)pk * J + pj) * kWave + lane];
+                    for (long e = 0; e < EV; ++e) {
+                        double aV[SF];
+                        if (vk <= vkb - B) {
+                            vkb = vk;
+                            const long st = vk - my_b;
+                            vfac = (my_b < B && st >= vk0)
+                                       ? step_factor<true>(p, coef_i, wv, st,
+                                                           my_j, my_m, cur_v)
+                                       : 0.0;
+                        }
+                        const int vbase = ((int)(vkb - vk) * J + vj) * M;
+                        if constexpr (MG > 0)
+                            assemble_reg<SF, MG>(aV, gf, vfac, vbase);
+                        else
+                            assemble<SF>(aV, lf, vfac, vbase, M, lane);
+                        double W[MA];
+                        W[0] = ypre;
+                        ypre = ypre2;
+                        next(pk, pj);
+                        ypre2 = yin[((size_t)pk * J + pj) * kWave + lane];
+                        // w_t = A^t y / t!, one 8-byte exchange per term
+#pragma unroll
+                        for (int t = 1; t < MA; ++t)
+                            W[t] = ell_product<SF>(aV, cf, W[t - 1], x0,
+                                                   lane, use_partner) *
+                                   (1.0 / t);
+                        // q_i = sum_t c_it w_t
+                        double Q[MA];
+#pragma unroll
+                        for (int i = 0; i < MA; ++i) {
+                            double c = 1.0 / (i + 1);
+                            double qv = c * W[0];
+#pragma unroll
+                            for (int t = 1; t + i < MA; ++t) {
+                                c = c * t / (i + t + 1);
+                                qv = fma(c, W[t], qv);
+                            }
+                            Q[i] = qv;
+                        }
+                        // wait for a free slot, fill it, publish it
+                        for (long poll = 0;
+                             !broken &&
+                             handed - __hip_atomic_load(
+                                          consumed, __ATOMIC_ACQUIRE,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP) >=
+                                 kRing;
+                             ++poll) {
+                            if (poll >= p.max_polls) {
+                                broken = true;
+                                if (lane == 0)
+                                    __hip_atomic_store(
+                                        overrun, 1, __ATOMIC_RELEASE,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                            } else {
+                                __builtin_amdgcn_s_sleep(1);
+                            }
+                        }
+                        double *hb = hand + slot * MA * kWave;
+#pragma unroll
+                        for (int i = 0; i < MA; ++i)
+                            hb[i * kWave + lane] = Q[i];
+                        if (++slot == kRing)
+                            slot = 0;
+                        ++handed;
+                        if (lane == 0)
+                            __hip_atomic_store(produced, (int)handed,
+                                               __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (++vj == J) {
+                            vj = 0;
+                            --vk;
+                        }
+                    }
+                }
+                __syncthreads();           // end of phase
+            }
+            return;
+        }
 
         for (int q = 0; q <= P; ++q) {
             const bool r_seg = q < P;      // recompute segment P - 1 - q
@@ -523,9 +650,15 @@ adjoint_kernel(AdjParams p)
                                       __HIP_MEMORY_SCOPE_WORKGROUP) >=
                              kRing;
                          ++poll) {
-                        if (poll == kMaxPolls)
+                        if (poll >= p.max_polls) {
                             broken = true;
-                        __builtin_amdgcn_s_sleep(1);
+                            if (lane == 0)
+                                __hip_atomic_store(
+                                    overrun, 1, __ATOMIC_RELEASE,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+                        } else {
+                            __builtin_amdgcn_s_sleep(1);
+                        }
                     }
                     double *hb = hand + slot * MA * kWave;
 #pragma unroll
@@ -584,17 +717,19 @@ adjoint_kernel(AdjParams p)
     // sums of <Abar, G_grad_gen>, reduced over the wavefront only when the
     // record changes.
     double amp_acc = 0.0;
+    long taken = 0;                     // exponentials taken over
+    int slot = 0;                       // ring slot of the next one
+    bool broken = false;
+    // after an overrun every record this instance adds to is poisoned
     auto flush = [&]() __attribute__((always_inline)) {
         if (cur_rec >= 0) {
-            const double v = wave_sum(amp_acc);
+            const double v =
+                broken ? __builtin_nan("") : wave_sum(amp_acc);
             if (lane == 0)
                 p.grad_seg[cur_rec] += v;
             amp_acc = 0.0;
         }
     };
-    long taken = 0;                     // exponentials taken over
-    int slot = 0;                       // ring slot of the next one
-    bool broken = false;
     const int JM = J * M;
     const int WB = JM + S;              // batch lanes per step
     const int BA = kWave / WB;          // steps per batch
@@ -648,9 +783,14 @@ adjoint_kernel(AdjParams p)
                                        __HIP_MEMORY_SCOPE_WORKGROUP) <=
                          taken;
                      ++poll) {
-                    if (poll == kMaxPolls)
+                    if (poll >= p.max_polls) {
                         broken = true;
-                    __builtin_amdgcn_s_sleep(1);
+                        if (lane == 0)
+                            __hip_atomic_store(overrun, 1, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    } else {
+                        __builtin_amdgcn_s_sleep(1);
+                    }
                 }
                 const double *hb = hand + slot * MA * kWave;
                 if (++slot == kRing)
@@ -662,6 +802,11 @@ adjoint_kernel(AdjParams p)
                 // the slot is free once its values are in registers
                 ssmx::wave_exchange();
                 __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+                // a producer that overran marks it before overwriting a
+                // slot, so values read after that carry the mark
+                if (__hip_atomic_load(overrun, __ATOMIC_ACQUIRE,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP))
+                    broken = true;
                 ++taken;
                 if (lane == 0)
                     __hip_atomic_store(consumed, (int)taken, __ATOMIC_RELEASE,
@@ -673,16 +818,31 @@ adjoint_kernel(AdjParams p)
 #pragma unroll
                 for (int sl = 0; sl < ST; ++sl)
                     T[sl] = 0.0;
+                // the stored-input variant always has the partner slot
+                const bool partner_slot = STORED || use_partner;
+                double gp[ST];          // the previous term's gathers
 #pragma unroll
                 for (int i = 0; i < MA; ++i) {
                     x1[lane] = u;
                     ssmx::wave_exchange();
                     double gu[ST];
                     gu[0] = u;
-                    gu[1] = use_partner ? swap_pair(u) : x1[ct[1]];
+                    gu[1] = partner_slot ? swap_pair(u) : x1[ct[1]];
 #pragma unroll
                     for (int sl = 2; sl < ST; ++sl)
                         gu[sl] = x1[ct[sl]];
+                    // The previous term's accumulation, issued while this
+                    // term's gathers are in flight (it needs none of them).
+                    // The empty asm keeps each accumulation here: T is only
+                    // read in conditional blocks below, and sinking the
+                    // chain there would keep every term's gathers alive.
+                    if (i > 0) {
+#pragma unroll
+                        for (int sl = 0; sl < ST; ++sl) {
+                            T[sl] = fma(gp[sl], Q[i - 1], T[sl]);
+                            asm volatile("" : "+v"(T[sl]));
+                        }
+                    }
                     ssmx::wave_exchange();
                     double u0 = a_t[0] * gu[0];
                     double u1 = 0.0;
@@ -695,14 +855,14 @@ adjoint_kernel(AdjParams p)
                     }
                     u = (u0 + u1) * (1.0 / (i + 1));
                     lam_new += u;
-                    // The empty asm keeps each accumulation here: T is only
-                    // read in conditional blocks below, and sinking the
-                    // chain there would keep every term's gathers alive.
 #pragma unroll
-                    for (int sl = 0; sl < ST; ++sl) {
-                        T[sl] = fma(gu[sl], Q[i], T[sl]);
-                        asm volatile("" : "+v"(T[sl]));
-                    }
+                    for (int sl = 0; sl < ST; ++sl)
+                        gp[sl] = gu[sl];
+                }
+#pragma unroll
+                for (int sl = 0; sl < ST; ++sl) {
+                    T[sl] = fma(gp[sl], Q[MA - 1], T[sl]);
+                    asm volatile("" : "+v"(T[sl]));
                 }
                 lam = lam_new;
 
@@ -791,16 +951,16 @@ adjoint_kernel(AdjParams p)
         p.lambda_out[(size_t)inst * n + lane] = lam;
 }
 
-template <int SF, int ST, int MG = 0>
+template <int SF, int ST, int MG = 0, int MA = kMaxAdj, bool STORED = false>
 int launch(const AdjParams &p, hipStream_t stream)
 {
     constexpr int SFP = SF + (SF & 1);
     constexpr int STP = ST + (ST & 1);
     const int M = p.d.n_gen;
-    // exchange buffers, handoff ring, staged values, two int counters
+    // exchange buffers, handoff ring, staged values, three int counters
     const size_t staged = MG == 0 ? (size_t)M * (SFP + STP) * kWave : 0;
     const size_t lds = (staged + 3 * kWave + (size_t)kRing * kMaxAdj * kWave +
-                        1) *
+                        2) *
                        sizeof(double);
     if (lds > 160 * 1024)
         return set_error(SSMX_EINVAL,
@@ -808,13 +968,15 @@ int launch(const AdjParams &p, hipStream_t stream)
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&adjoint_kernel<SF, ST, MG>),
+            reinterpret_cast<const void *>(
+                &adjoint_kernel<SF, ST, MG, MA, STORED>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
-    hipLaunchKernelGGL((adjoint_kernel<SF, ST, MG>), dim3(p.d.n_inst),
+    hipLaunchKernelGGL((adjoint_kernel<SF, ST, MG, MA, STORED>),
+                       dim3(p.d.n_inst),
                        dim3(2 * kWave), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -829,6 +991,18 @@ constexpr int kPairs[][2] = {{4, 4},   {4, 7},   {7, 7},   {4, 10},
                              {7, 10},  {10, 10}, {7, 16},  {10, 16},
                              {16, 16}, {16, 24}, {24, 24}, {32, 32}};
 
+// Stored-input variants: the planned adjoint degree rounded up to 8, 9 or
+// 10.
+template <int SF, int ST, int MG>
+int launch_stored(const AdjParams &p, hipStream_t s)
+{
+    if (p.d.taylor_adj <= 8)
+        return launch<SF, ST, MG, 8, true>(p, s);
+    if (p.d.taylor_adj == 9)
+        return launch<SF, ST, MG, 9, true>(p, s);
+    return launch<SF, ST, MG, 10, true>(p, s);
+}
+
 int dispatch(const AdjParams &p, hipStream_t s)
 {
     int pick = -1;
@@ -837,6 +1011,23 @@ int dispatch(const AdjParams &p, hipStream_t s)
             pick = i;
             break;
         }
+    if (p.d.flags & SSMX_STORE_INPUTS) {
+        // compiled for two or three register-held matrices, at most 7
+        // forward and 10 adjoint slots
+        const int M = p.d.n_gen;
+        if ((M == 2 || M == 3) && pick >= 0 && pick <= 4) {
+            const bool ten = p.d.slots_t > 7;
+            if (M == 2)
+                return ten ? launch_stored<7, 10, 2>(p, s)
+                           : launch_stored<7, 7, 2>(p, s);
+            return ten ? launch_stored<7, 10, 3>(p, s)
+                       : launch_stored<7, 7, 3>(p, s);
+        }
+        return set_error(SSMX_EINVAL,
+                         "stored-input adjoint needs 2 or 3 matrices and at "
+                         "most 7 / 10 slots (have %d, %d / %d)", M,
+                         p.d.slots, p.d.slots_t);
+    }
     // two or three matrices: generator values in registers
     const int M = p.d.n_gen;
     if (M == 2 || M == 3) {
@@ -909,11 +1100,16 @@ int validate(const ssmx_adjoint_desc *d)
     if (d->n_obs < 1 || d->n_obs > SSMX_MAX_OBS)
         return set_error(SSMX_EINVAL, "n_obs=%d outside 1..%d", d->n_obs,
                          SSMX_MAX_OBS);
-    if (d->flags & ~SSMX_COEF_WAVE)
+    if (d->flags & ~(SSMX_COEF_WAVE | SSMX_STORE_INPUTS))
         return set_error(SSMX_EINVAL,
-                         "flags=%d: the adjoint supports SSMX_COEF_WAVE only "
-                         "(non-rotating frame, integral readout)",
+                         "flags=%d: the adjoint supports SSMX_COEF_WAVE and "
+                         "SSMX_STORE_INPUTS only (non-rotating frame, "
+                         "integral readout)",
                          d->flags);
+    if ((d->flags & SSMX_STORE_INPUTS) && d->n > SSMX_MAX_N)
+        return set_error(SSMX_EINVAL,
+                         "stored inputs are supported for n <= %d",
+                         SSMX_MAX_N);
     if (d->first_out < 0)
         return set_error(SSMX_EINVAL, "first_out=%d < 0", d->first_out);
     if (d->partner != 0 && d->partner != 1)
@@ -983,6 +1179,10 @@ int ssmx_adjoint_workspace_bytes(const ssmx_adjoint_desc *desc,
                  (uint64_t)desc->n_exp * np * sizeof(double);
         return SSMX_OK;
     }
+    if (desc->flags & SSMX_STORE_INPUTS) {
+        *bytes = 0;                     // the inputs come from the forward
+        return SSMX_OK;
+    }
     *bytes = (uint64_t)desc->n_inst * 2u * (uint64_t)desc->ckpt_every *
              (uint64_t)desc->n_exp * kWave * sizeof(double);
     return SSMX_OK;
@@ -1003,9 +1203,10 @@ int ssmx_adjoint(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
         return rc;
     const bool wave_mode = (desc->flags & SSMX_COEF_WAVE) != 0;
     const bool wide = desc->n > SSMX_MAX_N;
+    const bool stored = (desc->flags & SSMX_STORE_INPUTS) != 0;
     if (!ell_col || !ell_val || !ell_col_t || (!ell_val_t && !wide) ||
         !cf_weights || !cf_nodes || !coef || !ckpt || !obs || !seed ||
-        !workspace)
+        (!workspace && !stored))
         return set_error(SSMX_EINVAL, "null required pointer");
     if (desc->n_theta > 0 && grad_theta && !dval_t)
         return set_error(SSMX_EINVAL, "n_theta > 0 needs dval_t");
@@ -1037,6 +1238,13 @@ int ssmx_adjoint(const ssmx_adjoint_desc *desc, const int32_t *ell_col,
     p.grad_theta = desc->n_theta > 0 ? grad_theta : nullptr;
     p.grad_seg = desc->grad_gen >= 0 ? grad_seg : nullptr;
     p.lambda_out = lambda_out;
+    p.max_polls = kMaxPolls;
+    if (const char *env = std::getenv("SSMX_ADJOINT_MAX_POLLS")) {
+        char *end = nullptr;
+        const long v = std::strtol(env, &end, 10);
+        if (end != env && v >= 0 && v < kMaxPolls)
+            p.max_polls = v;
+    }
     hipStream_t s = static_cast<hipStream_t>(stream);
 
     return dispatch(p, s);

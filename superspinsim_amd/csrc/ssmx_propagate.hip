@@ -133,7 +133,7 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
 // columns.  NPAIR: leading slot pairs fetched whole (ell_apply).  MG > 0:
 // the MG (= M) generator rows are read from LDS once into registers;
 // MG < 0: the first -MG rows are, the others are read at each exponent.
-template <int SLOTS, int NPAIR, int TAY, int MG = 0>
+template <int SLOTS, int NPAIR, int TAY, int MG = 0, bool STORE = false>
 __device__ __forceinline__ void propagate_instance(const Params &p,
                                                    const double *vals,
                                                    double *xbuf, int M,
@@ -209,7 +209,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
 
     // Checkpoints: with ckpt_every = K > 0, state_out receives the state
     // before steps 0, K, 2K, ... instead of the final state.
-    const int ckpt_every = p.state_out ? d.ckpt_every : 0;
+    const int ckpt_every = (!STORE && p.state_out) ? d.ckpt_every : 0;
     const long n_ckpt = ckpt_every > 0 ? (total + ckpt_every - 1) / ckpt_every
                                        : 0;
     int ckpt_count = 0;
@@ -252,6 +252,13 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                     for (int sl = 0; sl < SLOTS; ++sl)
                         a[sl] *= inv_sub;
                 }
+                if constexpr (STORE) {
+                    // SSMX_STORE_INPUTS: every exponential's input, for the
+                    // adjoint (stride 64 per exponential)
+                    p.state_out[(((size_t)inst * total + step) * J + j) *
+                                    kWave +
+                                lane] = x;
+                }
                 // the adjoint's recomputation repeats these roundings
                 for (int sub = 0; sub < d.substeps; ++sub)
                     x = taylor_action<SLOTS, NPAIR, TAY>(
@@ -292,7 +299,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                 p.out[(size_t)inst * d.n_obs + 1] = v1 + off1;
         }
     }
-    if (p.state_out && ckpt_every == 0 && lane < n)
+    if (!STORE && p.state_out && ckpt_every == 0 && lane < n)
         p.state_out[(size_t)inst * n + lane] = x;
 }
 
@@ -301,12 +308,12 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
 // per-term exits for degrees up to kUnrolledMax (SLOTS <= 16), else the
 // runtime loop.  Separate kernels keep each chain's register allocation
 // apart (both in one kernel ran every degree 1-2 % slower).
-template <int SLOTS, int NPAIR, int CHAIN, int MG>
+template <int SLOTS, int NPAIR, int CHAIN, int MG, bool STORE = false>
 __device__ __forceinline__ void propagate_instance_any(const Params &p, const double *vals, double *xbuf, int M, int inst, const int (&col)[SLOTS])
 {
     if constexpr (CHAIN > 0) {
-        propagate_instance<SLOTS, NPAIR, CHAIN, MG>(p, vals, xbuf, M, inst,
-                                                   col);
+        propagate_instance<SLOTS, NPAIR, CHAIN, MG, STORE>(p, vals, xbuf, M,
+                                                          inst, col);
     } else {
         if constexpr (SLOTS <= 16) {
             if (p.d.taylor >= 1 && p.d.taylor <= kUnrolledMax) {
@@ -318,7 +325,7 @@ __device__ __forceinline__ void propagate_instance_any(const Params &p, const do
     }
 }
 
-template <int SLOTS, int CHAIN, int MG>
+template <int SLOTS, int CHAIN, int MG, bool STORE = false>
 // At most 128 VGPRs: four wavefronts per SIMD (4096 instances fill the
 // device once).
 __global__ __launch_bounds__(kWave *kWavesPerBlock)
@@ -374,19 +381,19 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         constexpr int kAll = (SLOTS - 2) / 2;
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
-                propagate_instance_any<SLOTS, kAll, CHAIN, MG>(p, lds, xbuf, M, inst, col);
+                propagate_instance_any<SLOTS, kAll, CHAIN, MG, STORE>(p, lds, xbuf, M, inst, col);
                 return;
             }
         }
         if (np >= 1) {
-            propagate_instance_any<SLOTS, 1, CHAIN, MG>(p, lds, xbuf, M, inst, col);
+            propagate_instance_any<SLOTS, 1, CHAIN, MG, STORE>(p, lds, xbuf, M, inst, col);
             return;
         }
     }
-    propagate_instance_any<SLOTS, 0, CHAIN, MG>(p, lds, xbuf, M, inst, col);
+    propagate_instance_any<SLOTS, 0, CHAIN, MG, STORE>(p, lds, xbuf, M, inst, col);
 }
 
-template <int SLOTS, int CHAIN, int MG>
+template <int SLOTS, int CHAIN, int MG, bool STORE = false>
 int launch_chain(const Params &p, hipStream_t stream)
 {
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
@@ -400,14 +407,16 @@ int launch_chain(const Params &p, hipStream_t stream)
                          lds);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&propagate_kernel<SLOTS, CHAIN, MG>),
+            reinterpret_cast<const void *>(
+                &propagate_kernel<SLOTS, CHAIN, MG, STORE>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess)
             return set_error(SSMX_ELAUNCH, "hipFuncSetAttribute: %s",
                              hipGetErrorString(e));
     }
     const int blocks = (p.d.n_inst + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((propagate_kernel<SLOTS, CHAIN, MG>), dim3(blocks),
+    hipLaunchKernelGGL((propagate_kernel<SLOTS, CHAIN, MG, STORE>),
+                       dim3(blocks),
                        dim3(kWave * kWavesPerBlock), lds, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -421,6 +430,16 @@ template <int SLOTS>
 int launch(const Params &p, hipStream_t stream)
 {
     const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
+    if (p.d.flags & SSMX_STORE_INPUTS) {
+        // the adjoint's forward: degree 11, at most 8 slots (validated)
+        if constexpr (SLOTS <= 8) {
+            if (M == 2)
+                return launch_chain<SLOTS, kUnrolledTaylor, 2, true>(p,
+                                                                    stream);
+            return launch_chain<SLOTS, kUnrolledTaylor, -2, true>(p, stream);
+        }
+        return set_error(SSMX_EINVAL, "SSMX_STORE_INPUTS needs slots <= 8");
+    }
     if constexpr (SLOTS <= 8) {
         // two generator rows in registers: no LDS reads in the assembly
         if (p.d.taylor == kUnrolledTaylor && M == 2)
@@ -476,6 +495,18 @@ int validate(const ssmx_propagate_desc *d)
     if ((d->flags & SSMX_READ_FULL) && (d->flags & SSMX_READ_INTEGRAL))
         return set_error(SSMX_EINVAL,
                          "READ_FULL and READ_INTEGRAL are exclusive");
+    if (d->flags & SSMX_STORE_INPUTS) {
+        const int Mg = ((d->flags & SSMX_ROTATING) ? d->n_node : 1) *
+                       d->n_gen;
+        if (d->n > SSMX_MAX_N || (d->flags & SSMX_ROTATING) ||
+            d->substeps != 1 || d->ckpt_every != 0 ||
+            d->taylor != kUnrolledTaylor || d->slots > 8 || Mg < 2)
+            return set_error(SSMX_EINVAL,
+                             "SSMX_STORE_INPUTS needs n <= 64, the lab "
+                             "frame, substeps 1, ckpt_every 0, Taylor degree "
+                             "%d, at most 8 slots and 2 or more matrices",
+                             kUnrolledTaylor);
+    }
     if (!(d->h > 0.0) || !(d->dt > 0.0))
         return set_error(SSMX_EINVAL, "h and dt must be positive");
     if (d->first_out < 0)
@@ -554,6 +585,8 @@ int ssmx_propagate(const ssmx_propagate_desc *desc, const int32_t *ell_col,
         return set_error(SSMX_EINVAL, "waveform mode needs segments");
     if (!(desc->flags & SSMX_READ_FULL) && !obs)
         return set_error(SSMX_EINVAL, "obs readout needs obs rows");
+    if ((desc->flags & SSMX_STORE_INPUTS) && !state_out)
+        return set_error(SSMX_EINVAL, "SSMX_STORE_INPUTS needs state_out");
 
     Params p;
     p.d = *desc;

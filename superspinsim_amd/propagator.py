@@ -912,7 +912,8 @@ class Propagator:
     def run(self, *, n_inst, fine, n_out, h, t0, dt, rho0, coef=None,
             coef_inst_stride=0, waves=None, segments=None, taylor=None,
             substeps=1, read="full", obs=None, obs_offset=None, out=None,
-            state_out=None, ckpt_every=0, first_out=0, stream=None):
+            state_out=None, ckpt_every=0, first_out=0, stream=None,
+            store_inputs=False):
         """Launch one propagation; returns the output tensor.
 
         ``rho0`` (n_inst, n) device tensor.  Coefficients: ``coef`` device
@@ -942,6 +943,13 @@ class Propagator:
                 raise ValueError(f"unknown readout {read!r}")
         if waves is not None:
             flags |= _lib.SSMX_COEF_WAVE
+        if store_inputs:
+            # every exponential's input into state_out, for the adjoint
+            flags |= _lib.SSMX_STORE_INPUTS
+            want = (n_inst, n_out * fine, self.J, _lib.MAX_N)
+            if state_out is None or tuple(state_out.shape) != want:
+                raise ValueError(f"stored inputs need a state_out buffer of "
+                                 f"shape {want}")
         if out is None:
             out = torch.empty(shape, dtype=torch.float64, device="cuda")
         if taylor is None:

@@ -250,3 +250,70 @@ def test_autograd_function(workload, grads):
     size = (np.abs(w[:, None]) * np.abs(res["theta"])).sum(axis=0)
     got = theta.grad.numpy()
     assert np.all(np.abs(got - want) <= 1e-6 * size)
+
+
+@pytest.mark.parametrize("adjoint_tolerance", [1e-13, 1e-12])
+def test_stored_inputs_vs_recomputation(gpu, adjoint_tolerance):
+    """The backward from stored exponential inputs (no recomputation)
+    against the checkpointed one (adjoint degree 10): bit for bit at the
+    same degree; at the default tolerance (degree 9) within 1e-8 of each
+    parameter's largest gradient."""
+    res = {}
+    for store in (False, True):
+        wl = configs.odmr7_gradient(
+            count=COUNT, segments=SEGMENTS, time_end=T_END,
+            checkpoint_every=700, frequencies=FREQS,
+            adjoint_tolerance=adjoint_tolerance, store_inputs=store)
+        pb = wl.problem
+        out, state = pb.forward(wl.theta, wl.amplitudes)
+        assert state["stored"] == store
+        r = pb.backward(state, np.ones((COUNT, 1)), lambda_out=True)
+        res[store] = ({k: v.cpu().numpy() for k, v in r.items()},
+                      out.cpu().numpy(), int(state["taylor_adj"]))
+        print(f"store={store}: adjoint degree {state['taylor_adj']}",
+              flush=True)
+    (a, fa, ma), (b, fb, mb) = res[False], res[True]
+    assert np.array_equal(fa, fb)
+    if mb >= 10:
+        for k in ("theta", "records", "amplitudes", "lambda"):
+            assert np.array_equal(a[k], b[k]), k
+    else:
+        # relative to each gradient, with a floor on the parameter's natural
+        # scale F / theta_scale for components that cancel (dF/dBy is zero
+        # by symmetry with the field along z and the drive along x)
+        F = fa[:, 0]
+        err = np.abs(a["theta"] - b["theta"])
+        floor = 1e-9 * np.abs(F)[:, None] / configs.THETA_SCALE[None, :]
+        for k, name in enumerate(configs.THETA_NAMES):
+            print(f"  degree {mb}: {name:10s} |grad| {np.abs(a['theta'][:, k]).max():.3e}"
+                  f" err {err[:, k].max():.2e} floor {floor[:, k].max():.2e}")
+        assert np.all(err <= 1e-8 * np.abs(a["theta"]) + floor)
+        size = np.abs(a["amplitudes"]).max()
+        assert np.max(np.abs(a["amplitudes"] - b["amplitudes"])) <= \
+            1e-8 * size
+        assert np.max(np.abs(a["lambda"] - b["lambda"])) <= \
+            1e-12 * np.max(np.abs(a["lambda"]))
+
+
+@pytest.mark.parametrize("store", [False, True])
+def test_handoff_overrun_poisons_every_gradient(gpu, monkeypatch, store):
+    """A handoff wait that gives up (bound forced to 0 polls from the host;
+    every kernel still exits normally) must turn every output of the
+    affected instances into NaN, amplitude records included."""
+    wl = configs.odmr7_gradient(count=COUNT, segments=SEGMENTS,
+                                time_end=0.02e-6, checkpoint_every=300,
+                                frequencies=FREQS, store_inputs=store)
+    pb = wl.problem
+    out, state = pb.forward(wl.theta, wl.amplitudes)
+    monkeypatch.setenv("SSMX_ADJOINT_MAX_POLLS", "0")
+    r = pb.backward(state, np.ones((COUNT, 1)), lambda_out=True)
+    theta = r["theta"].cpu().numpy()
+    lam = r["lambda"].cpu().numpy()
+    rec = r["records"].cpu().numpy()
+    monkeypatch.delenv("SSMX_ADJOINT_MAX_POLLS")
+    assert np.all(np.isnan(theta)) and np.all(np.isnan(lam))
+    touched = np.asarray(state["tag_index"]) >= 0
+    assert np.all(np.isnan(rec[touched]))
+    # the bound restored: finite again
+    r = pb.backward(state, np.ones((COUNT, 1)))
+    assert np.all(np.isfinite(r["theta"].cpu().numpy()))
