@@ -78,6 +78,9 @@ def parse():
                    help="run only the gradient leg (profiling)")
     p.add_argument("--adjoint-instances", type=int, default=1024)
     p.add_argument("--adjoint-steps", type=int, default=2)
+    p.add_argument("--adjoint-recompute", action="store_true",
+                   help="gradient leg with checkpoints and recomputation "
+                        "instead of stored exponential inputs")
     p.add_argument("--no-protocols", action="store_true",
                    help="skip the uneven-protocol (work-queue) leg")
     p.add_argument("--protocols-only", action="store_true")
@@ -161,7 +164,9 @@ def adjoint_leg(args):
     import numpy as np
     import torch
     from superspinsim_amd import configs
-    wl = configs.odmr7_gradient(count=args.adjoint_instances)
+    wl = configs.odmr7_gradient(
+        count=args.adjoint_instances,
+        store_inputs=False if args.adjoint_recompute else "auto")
     pb = wl.problem
     t_setup = time.perf_counter()
     state = pb.setup(wl.theta, wl.amplitudes)
