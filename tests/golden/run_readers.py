@@ -42,10 +42,14 @@ def main(path="test-results.xml"):
               sorted(GOLDEN.items())]
     checks += [(name, _check(g, flags)) for name, (g, flags) in
                sorted(READERS.items())]
-    report = golden.run_checks(checks, suite="superspinsim_amd golden")
-    for name, r in report["superspinsim_amd golden"].items():
-        print(f"{name}: {r['outcome']} ({r['time']:.1f} s) "
-              f"{r.get('error', r['out'].strip())}", flush=True)
+    suite = "superspinsim_amd golden"
+    report = {suite: {}}
+    for check in checks:
+        one = golden.run_checks([check], suite=suite)[suite]
+        report[suite].update(one)
+        for name, r in one.items():
+            print(f"{name}: {r['outcome']} ({r['time']:.1f} s) "
+                  f"{r.get('error', r['out'].strip())}", flush=True)
     failed = golden.write_junit(report, path)
     print(f"{len(checks) - failed} passed, {failed} failed -> {path}")
     return 1 if failed else 0
