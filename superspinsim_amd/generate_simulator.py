@@ -99,6 +99,24 @@ def tabulate(sampler, n_gen, n_out, fine, nodes, time_start, time_step):
     return coef.reshape(n_out, fine, S, n_gen)
 
 
+def plan_steps(flat, gen_norms, count=48):
+    """Steps whose exponents plan the Taylor degree.
+
+    ``flat`` (steps, S, K) holds the node coefficients of every step and
+    ``gen_norms`` the 1-norms of the K generators they multiply ((K,), or
+    (S, K) when each node has its own matrices).  Picks the ``count`` steps
+    with the largest bound sum_{s,k} |c[s, k]| ||G_k||_1 on their
+    exponent's norm, so that a short spike on a generator of large norm is
+    planned for even where other coefficients are larger in magnitude, and
+    ``count`` steps spread over the run.
+    """
+    bound = (np.abs(flat) * np.asarray(gen_norms, dtype=_f64)).sum(axis=(1, 2))
+    count = min(count, flat.shape[0])
+    return np.unique(np.concatenate([
+        np.argsort(bound, kind="stable")[-count:],
+        np.linspace(0, flat.shape[0] - 1, count).astype(np.int64)]))
+
+
 def generate_simulator(
         sampler, generators, vectorisation_map=None, use_cuda=True,
         datatype=np.float64, use_residual=True, number_of_quartic_repeats=35,
@@ -209,13 +227,8 @@ def generate_simulator(
             ones + [coef[..., varying]], axis=-1))
         prop = Propagator(mats, weights, nodes, rotating=use_rotating,
                           eig=eig)
-        # exponents of actual steps: the 48 with the largest coefficient
-        # magnitudes and 48 spread over the run
         flat = coef.reshape(-1, coef.shape[2], coef.shape[3])  # (N*F, S, K)
-        mag = np.abs(flat).sum(axis=(1, 2))
-        pick = np.unique(np.concatenate([
-            np.argsort(mag)[-48:],
-            np.linspace(0, flat.shape[0] - 1, 48).astype(np.int64)]))
+        pick = plan_steps(flat, prop.gen_norms)
         m, s = prop.plan(flat[pick], h, tol=tolerance)
         if verbose:
             print(f"Taylor degree {m}, {s} substeps, {prop.ell.slots} ELL "

@@ -12,6 +12,7 @@ from golden.cases import GOLDEN, READERS, case, flatten_density, load_csv
 from oracle.reference_simulate import generate_simulator as oracle_gen
 from superspinsim_amd.generate_simulator import generate_simulator
 from superspinsim_amd.sugar import prepare_simspins
+from test_plan_steps import spiky_qubit
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +92,21 @@ def test_reader_variants_vs_oracle(gpu, name):
     err = np.max(np.abs(d_dev - d_ref)) / scale
     print(f"{name}: rel err {err:.3e}")
     assert err <= TOL
+
+
+def test_spiky_sampler_vs_oracle(gpu):
+    """A 0.5 ns, 50 mT spike on the x field inside a 100 ns run whose light
+    coefficient (on the qubit's all-zero optical generator) varies more in
+    magnitude: the Taylor plan must cover the spike steps (their exponents
+    are 100x the others'), or their truncation error shows at ~1e-6."""
+    sampler, gens, vmap, kw, rho = spiky_qubit()
+    t_dev, d_dev = generate_simulator(sampler, gens, vmap, verbose=True,
+                                      **kw)(rho, 0, 100e-9, 1e-9)
+    t_ref, d_ref = oracle_gen(sampler, gens, vmap, backend="c", **kw)(
+        rho, 0, 100e-9, 1e-9)
+    assert np.array_equal(t_dev, t_ref)
+    err = np.max(np.abs(d_dev - d_ref)) / np.max(np.abs(d_ref))
+    print(f"spiky sampler: rel err {err:.3e}")
+    assert err <= TOL
+    # the spike flips a visible part of the population
+    assert np.max(np.abs(d_ref[45, 1, 1] - d_ref[35, 1, 1])) > 1e-3
