@@ -259,9 +259,9 @@ class GradientProblem:
         prop = sim._propagator(active, const, self.dt)
         h = self.dt / self.F
         samples = sim.coefficient_box(waves, program)
-        m, _ = prop.plan(samples, h, tol=self.tolerance, max_substeps=1)
-        ma, _ = prop.plan(samples, h, tol=self.adjoint_tolerance,
-                          max_substeps=1, max_taylor=_lib.ADJ_MAX_TAYLOR)
+        m, ma = prop.plan_degrees(
+            samples, h, (self.tolerance, self.adjoint_tolerance),
+            max_taylor=(_lib.MAX_TAYLOR, _lib.ADJ_MAX_TAYLOR))
 
         # derivative tables of the folded constant matrix
         dgen = self.derivative_generators(theta) \
@@ -284,8 +284,8 @@ class GradientProblem:
 
         kwaves = BatchSimulator.kernel_waves(waves, program)
         tagged = sorted({k for k, w in enumerate(program)
-                         if any(p[0] >= 0 for wv in kwaves
-                                for p in wv.params[k])})
+                         if any(np.any(wv.segment_arrays(k)[1] >= 0)
+                                for wv in kwaves)})
         if len(tagged) > 1:
             raise NotImplementedError(
                 "amplitude parameters on more than one generator")

@@ -613,6 +613,45 @@ def taylor_plan(norm, tol=1e-17, max_taylor=_lib.MAX_TAYLOR):
     return best
 
 
+def taylor_degrees_measured(exponents, tols, safety=8.0, max_taylor=None):
+    """Least degree m with one substep meeting each tolerance of ``tols``.
+
+    One pass of the powers of the sampled exponents gives the degree
+    ``taylor_plan_measured(exponents, tol, safety, max_taylor=mt,
+    max_substeps=1)`` returns for every ``tol`` of ``tols`` (and ``mt`` of
+    ``max_taylor``, a sequence, or ``_lib.MAX_TAYLOR`` for all).
+    """
+    tols = [float(t) for t in tols]
+    if max_taylor is None:
+        max_taylor = [_lib.MAX_TAYLOR] * len(tols)
+    max_taylor = [int(m) for m in max_taylor]
+    exps = [np.asarray(a, dtype=_f64) for a in exponents]
+    norms = [float(np.abs(b).sum(axis=0).max()) for b in exps]
+    powers = [b.copy() for b in exps]
+    degrees = [None] * len(tols)
+    fact = 1.0
+    for m in range(1, max(max_taylor) + 1):
+        fact *= (m + 1)
+        worst = 0.0
+        for k, b in enumerate(exps):
+            powers[k] = powers[k] @ b          # B^(m+1), every sample
+            if norms[k] >= m + 2:
+                worst = math.inf
+            elif worst < math.inf:
+                pn = float(np.abs(powers[k]).sum(axis=0).max())
+                worst = max(worst, pn / fact / (1 - norms[k] / (m + 2)))
+        for i, tol in enumerate(tols):
+            if degrees[i] is None and m <= max_taylor[i] and \
+                    safety * worst <= tol:
+                degrees[i] = m
+        if all(d is not None for d in degrees):
+            break
+    for i, d in enumerate(degrees):
+        if d is None:
+            raise ValueError(f"no Taylor plan reaches tol={tols[i]}")
+    return degrees
+
+
 def taylor_plan_measured(exponents, tol=1e-17, safety=8.0,
                          max_taylor=_lib.MAX_TAYLOR, max_substeps=1024):
     """(m, s) from exact powers of representative exponents A.
@@ -636,12 +675,12 @@ def taylor_plan_measured(exponents, tol=1e-17, safety=8.0,
             fact *= (m + 1)
             worst = 0.0
             for k, b in enumerate(scaled):
-                powers[k] = powers[k] @ b          # B^(m+1)
+                powers[k] = powers[k] @ b          # B^(m+1), every sample
                 if norms[k] >= m + 2:
                     worst = math.inf
-                    break
-                pn = float(np.abs(powers[k]).sum(axis=0).max())
-                worst = max(worst, pn / fact / (1 - norms[k] / (m + 2)))
+                elif worst < math.inf:
+                    pn = float(np.abs(powers[k]).sum(axis=0).max())
+                    worst = max(worst, pn / fact / (1 - norms[k] / (m + 2)))
             if s * safety * worst <= tol:
                 if best is None or m * s < best[0] * best[1]:
                     best = (m, s)
@@ -894,6 +933,12 @@ class Propagator:
         return taylor_plan_measured(self.sample_exponents(coef_samples, h),
                                     tol=tol, max_substeps=max_substeps,
                                     max_taylor=max_taylor)
+
+    def plan_degrees(self, coef_samples, h, tols, max_taylor=None):
+        """One-substep Taylor degrees for several tolerances from one set
+        of sampled exponents (see taylor_degrees_measured)."""
+        return taylor_degrees_measured(self.sample_exponents(coef_samples, h),
+                                       tols, max_taylor=max_taylor)
 
     def norm_bound(self, coef_abs_max, h):
         """Bound on ||A_j||_1 from per-node maxima |c[s, g]| (S, K)."""
