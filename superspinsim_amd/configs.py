@@ -306,14 +306,18 @@ def odmr7_gradient(count=1024, segments=200, seed=3, time_end=2e-6,
     seg_len = time_end / segments
     dscale = drive_amplitude(1.0)
 
+    index = np.arange(segments)
+    starts = index * seg_len
+    ends = (index + 1) * seg_len
+
     def waveforms(amps):
+        # drive_amplitude of every segment amplitude (the same operations)
+        b1 = math.sqrt(2) * np.asarray(amps, dtype=np.float64) / GYRO_HZ_PER_T
         out = []
         for f in freqs:
             w = Waveform(5, base=[1.0, 0.0, 0.0, 0.0, base.excitation])
-            for k in range(segments):
-                w.add(1, k * seg_len, (k + 1) * seg_len,
-                      drive_amplitude(amps[k]), math.tau * f, 0.0,
-                      param=k, dparam=dscale)
+            w.add_segments(1, starts, ends, b1, math.tau * f, 0.0,
+                           param=index, dparam=dscale)
             out.append(w)
         return out
 

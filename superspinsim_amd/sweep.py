@@ -29,8 +29,20 @@ from superspinsim_amd.propagator import (
 _f64 = np.float64
 
 
+_NO_ROWS = np.zeros((0, 5), dtype=_f64)
+_NO_PARAMS = np.zeros(0, dtype=np.int64)
+_NO_DPARAMS = np.zeros(0, dtype=_f64)
+
+
 class Waveform:
-    """Coefficient program of one instance for ``n_gen`` generators."""
+    """Coefficient program of one instance for ``n_gen`` generators.
+
+    Each generator's segments are held as arrays: rows (t_start, t_end,
+    amp, omega, phase) in insertion order and, per row, a gradient tag
+    (parameter index or -1, d amp / d parameter).  ``segments`` and
+    ``params`` show them as lists of tuples; change them through ``add``,
+    ``add_segments``, ``set_segments`` or ``copy_segments``.
+    """
 
     def __init__(self, n_gen, base=None):
         if n_gen > _lib.MAX_GEN:
@@ -39,9 +51,18 @@ class Waveform:
         self.base = np.zeros(n_gen, dtype=_f64)
         if base is not None:
             self.base[:] = base
-        self.segments = [[] for _ in range(n_gen)]
-        # per segment: (parameter index or -1, d amp / d parameter)
-        self.params = [[] for _ in range(n_gen)]
+        # merged arrays per generator (never changed in place, so copies
+        # may share them) and rows added one at a time since the last merge
+        self._rows = [_NO_ROWS] * n_gen
+        self._param = [_NO_PARAMS] * n_gen
+        self._dparam = [_NO_DPARAMS] * n_gen
+        self._pending = [[] for _ in range(n_gen)]
+        self._amp_max = np.zeros(n_gen, dtype=_f64)
+        self._views = None
+
+    def _changed(self, g, amp_max):
+        self._amp_max[g] = amp_max
+        self._views = None
 
     def add(self, g, t_start, t_end, amp, omega=0.0, phase=0.0, param=-1,
             dparam=1.0):
@@ -50,10 +71,98 @@ class Waveform:
         ``param``/``dparam`` tag the segment for gradients: its ``amp``
         depends on parameter ``param`` with derivative ``dparam``.
         """
-        self.segments[g].append((float(t_start), float(t_end), float(amp),
-                                 float(omega), float(phase)))
-        self.params[g].append((int(param), float(dparam)))
+        self._pending[g].append((float(t_start), float(t_end), float(amp),
+                                 float(omega), float(phase), int(param),
+                                 float(dparam)))
+        self._changed(g, max(self._amp_max[g], abs(float(amp))))
         return self
+
+    def add_segments(self, g, t_start, t_end, amp, omega=0.0, phase=0.0,
+                     param=-1, dparam=1.0):
+        """``add`` for many segments of c_g at once: the arguments are
+        arrays (or scalars) broadcast to one length, segment k taking
+        element k of each."""
+        cols = np.broadcast_arrays(
+            *(np.asarray(v, dtype=_f64)
+              for v in (t_start, t_end, amp, omega, phase, dparam)),
+            np.asarray(param, dtype=np.int64))
+        rows, param_g, dparam_g = self.segment_arrays(g)
+        add = np.stack([c.ravel() for c in cols[:5]], axis=1)
+        self._rows[g] = np.concatenate([rows, add])
+        self._param[g] = np.concatenate([param_g, cols[6].ravel()])
+        self._dparam[g] = np.concatenate([dparam_g, cols[5].ravel()])
+        a = float(np.max(np.abs(add[:, 2]))) if add.shape[0] else 0.0
+        self._changed(g, max(self._amp_max[g], a))
+        return self
+
+    def set_segments(self, g, segments, params=None):
+        """Replace c_g's segments by ``segments`` ((m, 5) rows) and their
+        gradient tags by ``params`` ((m, 2) rows, untagged by default)."""
+        rows = np.array(segments, dtype=_f64).reshape(-1, 5)
+        if params is None:
+            param = np.full(rows.shape[0], -1, dtype=np.int64)
+            dparam = np.ones(rows.shape[0], dtype=_f64)
+        else:
+            tags = list(params)
+            if len(tags) != rows.shape[0]:
+                raise ValueError("one gradient tag per segment")
+            param = np.array([int(t[0]) for t in tags], dtype=np.int64)
+            dparam = np.array([float(t[1]) for t in tags], dtype=_f64)
+        self._rows[g], self._param[g], self._dparam[g] = rows, param, dparam
+        self._pending[g] = []
+        self._changed(g, float(np.max(np.abs(rows[:, 2])))
+                      if rows.shape[0] else 0.0)
+        return self
+
+    def copy_segments(self, g, other, other_g):
+        """c_g's segments and tags become those of ``other``'s c_other_g."""
+        rows, param, dparam = other.segment_arrays(other_g)
+        self._rows[g], self._param[g], self._dparam[g] = rows, param, dparam
+        self._pending[g] = []
+        self._changed(g, float(other._amplitude_max()[other_g]))
+        return self
+
+    def segment_arrays(self, g):
+        """(rows (m, 5), parameter index (m,), d amp / d parameter (m,)) of
+        c_g's segments in insertion order (shared; do not modify)."""
+        pend = self._pending[g]
+        if pend:
+            arr = np.array(pend, dtype=_f64).reshape(-1, 7)
+            self._rows[g] = np.concatenate([self._rows[g], arr[:, :5]])
+            self._param[g] = np.concatenate(
+                [self._param[g], np.array([r[5] for r in pend],
+                                          dtype=np.int64)])
+            self._dparam[g] = np.concatenate([self._dparam[g], arr[:, 6]])
+            self._pending[g] = []
+        return self._rows[g], self._param[g], self._dparam[g]
+
+    def segment_count(self, g):
+        return self._rows[g].shape[0] + len(self._pending[g])
+
+    def _lists(self):
+        if self._views is None:
+            segs, tags = [], []
+            for g in range(self.n_gen):
+                rows, param, dparam = self.segment_arrays(g)
+                segs.append(list(map(tuple, rows.tolist())))
+                tags.append(list(zip(param.tolist(), dparam.tolist())))
+            self._views = (segs, tags)
+        return self._views
+
+    @property
+    def segments(self):
+        """Per generator, the (t_start, t_end, amp, omega, phase) tuples."""
+        return self._lists()[0]
+
+    @property
+    def params(self):
+        """Per generator and segment, (parameter index or -1, d amp / d
+        parameter)."""
+        return self._lists()[1]
+
+    def _amplitude_max(self):
+        """max |amp| per generator (0 without segments)."""
+        return self._amp_max
 
     def tone(self, g, amp, frequency, t_start=-math.inf, t_end=math.inf,
              phase=0.0):
@@ -71,35 +180,27 @@ class Waveform:
 
     def bounds(self):
         """(lower, upper) bounds of c_g(t) over all t."""
-        lo = self.base.copy()
-        hi = self.base.copy()
-        for g, segs in enumerate(self.segments):
-            if segs:
-                a = max(abs(s[2]) for s in segs)
-                lo[g] -= a
-                hi[g] += a
-        return lo, hi
+        a = self._amplitude_max()
+        return self.base - a, self.base + a
 
     def abs_max(self):
-        """Upper bound of |c_g(t)| over all t."""
-        out = np.abs(self.base).copy()
-        for g, segs in enumerate(self.segments):
-            # segments of one generator do not overlap
-            if segs:
-                out[g] += max(abs(s[2]) for s in segs)
-        return out
+        """Upper bound of |c_g(t)| over all t (segments of one generator
+        do not overlap)."""
+        return np.abs(self.base) + self._amplitude_max()
 
     def shifted(self, offset):
         """The waveform seen from a clock started at ``offset``:
         ``shifted(T)(t) = self(t + T)`` up to rounding.  Segment windows move
         by -T and each tone's phase gains omega T (reduced mod 2 pi)."""
         out = Waveform(self.n_gen, self.base)
-        for g, segs in enumerate(self.segments):
-            for (ts, te, amp, om, ph), tag in zip(segs, self.params[g]):
-                out.segments[g].append(
-                    (ts - offset, te - offset, amp, om,
-                     math.remainder(ph + om * offset, math.tau)))
-                out.params[g].append(tag)
+        for g in range(self.n_gen):
+            rows, param, dparam = self.segment_arrays(g)
+            moved = rows.copy()
+            moved[:, 0] -= offset
+            moved[:, 1] -= offset
+            moved[:, 4] = [math.remainder(ph + om * offset, math.tau)
+                           for om, ph in rows[:, 3:5].tolist()]
+            out.set_segments(g, moved, zip(param.tolist(), dparam.tolist()))
         return out
 
     def sampler(self, rotating_offset=0):
@@ -117,37 +218,47 @@ WAVE_DTYPE = np.dtype([("base", _f64, _lib.MAX_GEN),
 def waveform_records(waves, n_gen, with_tags=False):
     """Host arrays matching ``ssmx_wave[]`` and ``ssmx_segment[]``.
 
-    With ``with_tags`` also returns, per segment record, the parameter
-    index (-1 if untagged) and d amp / d parameter.  Every instance gets
-    its own records (the adjoint accumulates per record).
+    Each generator's segments are stored in ascending (t_start, t_end, amp,
+    omega, phase) order; overlapping windows are refused.  With
+    ``with_tags`` also returns, per segment record, the parameter index (-1
+    if untagged) and d amp / d parameter.  Every instance gets its own
+    records (the adjoint accumulates per record).
     """
     recs = np.zeros(len(waves), dtype=WAVE_DTYPE)
     segs = []
     tags = []
+    count = 0
     for i, w in enumerate(waves):
         if w.n_gen != n_gen:
             raise ValueError("waveforms must cover the same generators")
         recs["base"][i, :n_gen] = w.base
         for g in range(n_gen):
-            order = sorted(range(len(w.segments[g])),
-                           key=lambda k: w.segments[g][k])
-            ordered = [w.segments[g][k] for k in order]
-            for a, b in zip(ordered, ordered[1:]):
-                if b[0] < a[1]:
-                    raise ValueError("overlapping segments on generator "
-                                     f"{g} of instance {i}")
-            recs["seg_begin"][i, g] = len(segs)
-            recs["seg_count"][i, g] = len(ordered)
-            segs.extend(ordered)
-            tags.extend(w.params[g][k] for k in order)
+            recs["seg_begin"][i, g] = count
+            arr, param, dparam = w.segment_arrays(g)
+            nseg = arr.shape[0]
+            recs["seg_count"][i, g] = nseg
+            if not nseg:
+                continue
+            # lexicographic order of the rows (np.lexsort: last key first);
+            # rows already in strictly ascending t_start keep their order
+            if nseg > 1 and not np.all(arr[1:, 0] > arr[:-1, 0]):
+                order = np.lexsort(arr.T[::-1])
+                arr, param, dparam = arr[order], param[order], dparam[order]
+            if np.any(arr[1:, 0] < arr[:-1, 1]):
+                raise ValueError("overlapping segments on generator "
+                                 f"{g} of instance {i}")
+            segs.append(arr)
+            if with_tags:
+                tags.append((param, dparam))
+            count += nseg
     if not segs:
-        segs = [(0.0, 0.0, 0.0, 0.0, 0.0)]
-        tags = [(-1, 0.0)]
-    seg_arr = np.array(segs, dtype=_f64).reshape(-1, 5)
+        segs = [np.zeros((1, 5), dtype=_f64)]
+        tags = [(np.array([-1], dtype=np.int64), np.zeros(1, dtype=_f64))]
+    seg_arr = np.concatenate(segs)
     if not with_tags:
         return recs, seg_arr
-    index = np.array([t[0] for t in tags], dtype=np.int64)
-    scale = np.array([t[1] for t in tags], dtype=_f64)
+    index = np.concatenate([t[0] for t in tags])
+    scale = np.concatenate([t[1] for t in tags])
     return recs, seg_arr, index, scale
 
 
@@ -308,7 +419,8 @@ class BatchSimulator:
         const = {}
         for g in active:
             bases = {float(w.base[g]) for w in waves}
-            if len(bases) == 1 and not any(w.segments[g] for w in waves):
+            if len(bases) == 1 and not any(w.segment_count(g)
+                                           for w in waves):
                 const[g] = bases.pop()
         program = ([None] if const else []) + \
             [g for g in active if g not in const]
@@ -344,8 +456,7 @@ class BatchSimulator:
             nw = Waveform(len(program), base)
             for k, g in enumerate(program):
                 if g is not None:
-                    nw.segments[k] = list(w.segments[g])
-                    nw.params[k] = list(w.params[g])
+                    nw.copy_segments(k, w, g)
             out.append(nw)
         return out
 
@@ -412,8 +523,7 @@ class BatchSimulator:
             for j, mw in enumerate(members):
                 g, slot = n_const + v, n_const + v * k + j
                 w.base[slot] = mw.base[g]
-                w.segments[slot] = list(mw.segments[g])
-                w.params[slot] = list(mw.params[g])
+                w.copy_segments(slot, mw, g)
         return w
 
     def run(self, waves, rho0, time_start, time_end, time_step, obs=None,
