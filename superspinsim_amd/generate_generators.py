@@ -847,22 +847,79 @@ def _coordinates(mats, ys, xs, cs):
     return np.where(cs == 0, picked.real, picked.imag)
 
 
+def _coordinate_rows(hilbert_size, valid_indices, use_hermitian):
+    """Coordinate count and, per entry (y, x) of a density matrix (index
+    y * h + x), the coordinate holding its real and its imaginary part
+    (-1 where the vectorisation keeps none)."""
+    h = hilbert_size
+    if use_hermitian:
+        ys, xs, cs = (valid_indices[:, 0], valid_indices[:, 1],
+                      valid_indices[:, 2])
+    else:
+        k = np.arange(2 * h * h)
+        cs, xs, ys = k % 2, (k // 2) % h, (k // 2) // h
+    flat = np.asarray(ys) * h + np.asarray(xs)
+    cs = np.asarray(cs)
+    real = np.full(h * h, -1, dtype=np.int64)
+    imag = np.full(h * h, -1, dtype=np.int64)
+    real[flat[cs == 0]] = np.nonzero(cs == 0)[0]
+    imag[flat[cs == 1]] = np.nonzero(cs == 1)[0]
+    return flat.size, real, imag
+
+
+def _sandwich_superoperator(pairs, valid_indices, use_hermitian):
+    """Real matrix, in the chosen basis, of rho -> sum_j A_j rho B_j.
+
+    Row-major, rho -> A rho B is the matrix K[(y, x), (a, b)] =
+    A[y, a] B[b, x]; only the products of non-zero entries are formed.  The
+    basis element of coordinate (y, x, c) is q e_(y,x) (q = 1, or i for
+    c = 1) plus, in the Hermitian basis off the diagonal, conj(q) e_(x,y);
+    row (y, x, c) of the result is the real (c = 0) or imaginary part of
+    entry (y, x) of the image.
+    """
+    h = pairs[0][0].shape[0]
+    dim, real, imag = _coordinate_rows(h, valid_indices, use_hermitian)
+    rows, cols, vals = [], [], []
+    for a, b in pairs:
+        ya, xa = np.nonzero(a)
+        yb, xb = np.nonzero(b)
+        rows.append((ya[:, None] * h + xb[None, :]).ravel())
+        cols.append((xa[:, None] * h + yb[None, :]).ravel())
+        vals.append((a[ya, xa][:, None] * b[yb, xb][None, :]).ravel())
+    row = np.concatenate(rows)
+    col = np.concatenate(cols)
+    val = np.concatenate(vals).astype(complex)
+    ca, cb = col // h, col % h
+    col_t = cb * h + ca
+    cases = [(real[col], 1.0, True), (imag[col], 1j, True)]
+    if use_hermitian:
+        off = ca != cb
+        cases += [(real[col_t], 1.0, off), (imag[col_t], -1j, off)]
+    out = np.zeros((dim, dim), dtype=_f64)
+    for k, q, use in cases:
+        v = val * q
+        for r, part in ((real[row], v.real), (imag[row], v.imag)):
+            m = use & (k >= 0) & (r >= 0)
+            np.add.at(out, (r[m], k[m]), part[m])
+    return out
+
+
 def _generate_von_neumann(operator, valid_indices, use_hermitian=True):
     """Real matrix of rho -> -i [H, rho] in the chosen basis."""
-    basis, ys, xs, cs = _basis(operator.shape[0], valid_indices, use_hermitian)
-    out = -1j * (operator @ basis - basis @ operator)
-    return np.ascontiguousarray(_coordinates(out, ys, xs, cs).T)
+    eye = np.eye(operator.shape[0])
+    return _sandwich_superoperator([(-1j * operator, eye),
+                                    (eye, 1j * operator)],
+                                   valid_indices, use_hermitian)
 
 
 def _generate_dissipator(operator, valid_indices, use_hermitian=True):
     """Real matrix of rho -> L rho L^+ - {L^+ L, rho} / 2."""
-    basis, ys, xs, cs = _basis(operator.shape[0], valid_indices, use_hermitian)
+    eye = np.eye(operator.shape[0])
     dag = operator.conj().T
     proj = dag @ operator
-    out = operator @ basis @ dag
-    out = out - 0.5 * (proj @ basis)
-    out = out - 0.5 * (basis @ proj)
-    return np.ascontiguousarray(_coordinates(out, ys, xs, cs).T)
+    return _sandwich_superoperator([(operator, dag), (-0.5 * proj, eye),
+                                    (eye, -0.5 * proj)],
+                                   valid_indices, use_hermitian)
 
 
 def _generate_superoperators(operator_dict, valid_indices, use_hermitian=True):
@@ -890,7 +947,11 @@ def _combine_superoperators(superoperators):
     combined = {}
 
     def accumulate(target, key, op):
-        target[key] = op.copy() if key not in target else target[key] + op
+        # the first term is copied, so the sums never write to an input
+        if key in target:
+            target[key] += op
+        else:
+            target[key] = op.copy()
 
     add = {}
     for label, op in superoperators.items():
