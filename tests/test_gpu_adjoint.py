@@ -152,7 +152,8 @@ def test_gradients_match_oracle():
         assert np.all(np.abs(kr - gro) <= 1e-8 * np.max(np.abs(gro)))
 
 
-SPEC_PARAMS = ("D_gs", "D_es", "Bz", "radiative", "isc_pm")
+#: every parameter (Bx and By are 0 at theta0, where F is even in them)
+SPEC_PARAMS = configs.THETA_NAMES
 
 
 def test_theta_gradient_central_fd_spec_step(workload, grads):

@@ -100,8 +100,9 @@ def test_spiky_sampler_vs_oracle(gpu):
     magnitude: the Taylor plan must cover the spike steps (their exponents
     are 100x the others'), or their truncation error shows at ~1e-6."""
     sampler, gens, vmap, kw, rho = spiky_qubit()
-    t_dev, d_dev = generate_simulator(sampler, gens, vmap, verbose=True,
-                                      **kw)(rho, 0, 100e-9, 1e-9)
+    t_dev, d_dev = generate_simulator(sampler, gens, vmap,
+                                      **dict(kw, verbose=True))(
+        rho, 0, 100e-9, 1e-9)
     t_ref, d_ref = oracle_gen(sampler, gens, vmap, backend="c", **kw)(
         rho, 0, 100e-9, 1e-9)
     assert np.array_equal(t_dev, t_ref)
