@@ -10,6 +10,12 @@ property.
   the trace's max |value|.  The oracle itself agrees with scipy ``expm``
   applied step by step to 5.5e-13 (7-level, 2.73 GHz) and 3.6e-14
   (3-level) over these horizons.
+* Oracle at full length for the pulsed protocol batch: the shortest
+  Hahn-echo and the shortest Ramsey protocol of the 16384-protocol batch,
+  whole (about 58k-65k steps, the last 50k the 1 us readout window),
+  through the persistent queue kernel and through the reference algorithm
+  with an output at every step, integrated over the readout window the same
+  way; 1e-10 relative.
 * Vectorisation identity: the 7-level model vectorised coherently
   (n = 49) and compressed (n = 19, coherences between incoherently coupled
   blocks dropped, which stay zero from the states used here) is the same
@@ -90,6 +96,42 @@ def test_full_length_chunked_3_level_vs_oracle(gpu):
     err = np.max(np.abs(got[7] - ref)) / np.max(np.abs(ref))
     print(f"  chunked vs oracle rel err {err:.3e}", flush=True)
     assert err <= TOL
+
+
+def test_full_length_protocols_vs_oracle(gpu):
+    from superspinsim_amd.protocols import trapezoid_window
+    full = configs.pulsed7()
+    echo = full.params["echo"]
+    picks = [int(np.flatnonzero(echo)[np.argmin(full.steps[echo])]),
+             int(np.flatnonzero(~echo)[np.argmin(full.steps[~echo])])]
+    waves = [full.waves[i] for i in picks]
+    batch = full.batch().prepare(waves, full.rho0, full.time_step,
+                                 full.steps[picks], full.read_from[picks],
+                                 full.obs)
+    print(f"{full.name}: protocols {picks}, steps "
+          f"{full.steps[picks].tolist()} on the GPU", flush=True)
+    out = batch.run().cpu().numpy()[:, 0]
+    base = configs.odmr7(1)
+    for k, i in enumerate(picks):
+        print(f"  protocol {i} (echo {bool(echo[i])}): oracle at every step",
+              flush=True)
+        sim = oracle_gen(None, full.generators, base.vmap, backend="c",
+                         number_of_exponentials=2, number_of_fine_divisions=1,
+                         vectorised_sampler=vectorised_sampler(full.waves[i]))
+        T = int(full.steps[i])
+        rho = np.zeros((7, 7))
+        rho[1, 1] = 1.0
+        _, dens = sim(rho, 0.0, T * full.time_step * (1 + 1e-12),
+                      full.time_step)
+        assert dens.shape[0] == T
+        pops = np.real(dens[:, 3, 3] + dens[:, 4, 4] + dens[:, 5, 5])
+        want = trapezoid_window(np.concatenate([[0.0], pops])[None],
+                                full.read_from[i:i + 1],
+                                full.steps[i:i + 1], full.time_step)[0]
+        err = abs(out[k] - want) / abs(want)
+        print(f"  integrated fluorescence {out[k]:.12e}, rel err {err:.3e}",
+              flush=True)
+        assert err <= TOL
 
 
 def test_odmr7_full_length_vectorisations_agree(gpu):
