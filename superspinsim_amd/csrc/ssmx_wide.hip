@@ -37,6 +37,9 @@
 
 namespace {
 
+// workgroups of at most this many wavefronts get 256 VGPRs per lane
+constexpr int kSmallBlockWaves = 4;
+
 using ssmx::node_time;
 using ssmx::set_error;
 using ssmx::wave_coef;
@@ -95,8 +98,14 @@ __device__ double wide_factor(const WParams &p, const double *coef_i,
     return d.h * cw;
 }
 
-template <int SLOTS>
-__global__ __launch_bounds__(kWave *kMaxBlockWaves) void wide_kernel(
+// WPB: most wavefronts per workgroup.  At the full kMaxBlockWaves four
+// wavefronts share a SIMD, so each gets 128 VGPRs; workgroups of at most
+// kSmallBlockWaves (one instance, n <= 256) keep two wavefronts per SIMD
+// and 256 VGPRs, where the 14- and 12-slot rows no longer spill.
+template <int SLOTS, int WPB = kMaxBlockWaves>
+__global__ __launch_bounds__(kWave *WPB)
+__attribute__((amdgpu_waves_per_eu(WPB <= kSmallBlockWaves ? 2 : 4)))
+void wide_kernel(
     WParams p, int W, int IPB)
 {
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
@@ -307,6 +316,34 @@ int launch(const WParams &p, hipStream_t stream)
                          "n=%d: generator values need %zu bytes of LDS "
                          "(> 160 KiB)",
                          p.d.n, lds_of(ipb_max));
+    // n <= 256: one instance per workgroup of at most kSmallBlockWaves
+    // wavefronts with 256 VGPRs, when two or more such workgroups are
+    // resident per CU (the same operations, so the same results)
+    if (W <= kSmallBlockWaves) {
+        const void *small = reinterpret_cast<const void *>(
+            &wide_kernel<SLOTS, kSmallBlockWaves>);
+        bool ok = true;
+        if (lds_of(1) > 64 * 1024)
+            ok = hipFuncSetAttribute(
+                     small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                     (int)lds_of(1)) == hipSuccess;
+        int nb = 0;
+        if (ok)
+            ok = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                     &nb, small, NP, lds_of(1)) == hipSuccess;
+        if (!ok)
+            (void)hipGetLastError();    // fall back to the full variant
+        if (ok && nb >= 2) {
+            hipLaunchKernelGGL((wide_kernel<SLOTS, kSmallBlockWaves>),
+                               dim3(p.d.n_inst), dim3(NP), lds_of(1), stream,
+                               p, W, 1);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess)
+                return set_error(SSMX_ELAUNCH, "wide propagate launch: %s",
+                                 hipGetErrorString(e));
+            return SSMX_OK;
+        }
+    }
     const void *kern = reinterpret_cast<const void *>(&wide_kernel<SLOTS>);
     if (lds_of(ipb_max) > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(
