@@ -40,7 +40,7 @@ class Waveform:
     Each generator's segments are held as arrays: rows (t_start, t_end,
     amp, omega, phase) in insertion order and, per row, a gradient tag
     (parameter index or -1, d amp / d parameter).  ``segments`` and
-    ``params`` show them as lists of tuples; change them through ``add``,
+    ``params`` show them as tuples of tuples; change them through ``add``,
     ``add_segments``, ``set_segments`` or ``copy_segments``.
     """
 
@@ -139,26 +139,27 @@ class Waveform:
     def segment_count(self, g):
         return self._rows[g].shape[0] + len(self._pending[g])
 
-    def _lists(self):
+    def _views_of(self):
         if self._views is None:
             segs, tags = [], []
             for g in range(self.n_gen):
                 rows, param, dparam = self.segment_arrays(g)
-                segs.append(list(map(tuple, rows.tolist())))
-                tags.append(list(zip(param.tolist(), dparam.tolist())))
-            self._views = (segs, tags)
+                segs.append(tuple(map(tuple, rows.tolist())))
+                tags.append(tuple(zip(param.tolist(), dparam.tolist())))
+            self._views = (tuple(segs), tuple(tags))
         return self._views
 
     @property
     def segments(self):
-        """Per generator, the (t_start, t_end, amp, omega, phase) tuples."""
-        return self._lists()[0]
+        """Per generator, the (t_start, t_end, amp, omega, phase) tuples
+        (immutable: change them through the methods above)."""
+        return self._views_of()[0]
 
     @property
     def params(self):
         """Per generator and segment, (parameter index or -1, d amp / d
-        parameter)."""
-        return self._lists()[1]
+        parameter) (immutable)."""
+        return self._views_of()[1]
 
     def _amplitude_max(self):
         """max |amp| per generator (0 without segments)."""

@@ -82,7 +82,7 @@ def test_set_and_copy_segments_equal_add():
     _same(want, copy)
     # untagged by default
     bare = Waveform(1).set_segments(0, per_gen[0][0])
-    assert bare.params[0] == [(-1, 1.0)] * 4
+    assert bare.params[0] == ((-1, 1.0),) * 4
     with pytest.raises(ValueError):
         Waveform(1).set_segments(0, per_gen[0][0], per_gen[0][1][:2])
 
@@ -97,8 +97,11 @@ def test_mixed_add_forms_keep_insertion_order():
                    dparam=[t[1] for t in tags[1:4]])
     for row, tag in zip(rows[4:], tags[4:]):
         w.add(0, *row, param=tag[0], dparam=tag[1])
-    assert w.segments[0] == [tuple(x) for x in rows]
-    assert w.params[0] == tags
+    assert w.segments[0] == tuple(tuple(x) for x in rows)
+    assert w.params[0] == tuple(tags)
+    # read-only views: mutation fails loudly instead of being ignored
+    with pytest.raises(AttributeError):
+        w.segments[0].append(rows[0])
     assert w.segment_count(0) == 6
     assert math.isclose(w.abs_max()[0], max(abs(x[2]) for x in rows))
 
@@ -118,9 +121,9 @@ def test_records_sorted_with_their_tags():
 def test_scalar_broadcast_and_empty_batch():
     w = Waveform(2)
     w.add_segments(1, [0.0, 1e-9], [1e-9, 2e-9], 0.25, omega=3.0)
-    assert w.segments[1] == [(0.0, 1e-9, 0.25, 3.0, 0.0),
-                             (1e-9, 2e-9, 0.25, 3.0, 0.0)]
-    assert w.params[1] == [(-1, 1.0), (-1, 1.0)]
+    assert w.segments[1] == ((0.0, 1e-9, 0.25, 3.0, 0.0),
+                             (1e-9, 2e-9, 0.25, 3.0, 0.0))
+    assert w.params[1] == ((-1, 1.0), (-1, 1.0))
     w.add_segments(0, [], [], [])
     assert w.segment_count(0) == 0 and w.abs_max()[0] == 0.0
 
