@@ -298,7 +298,14 @@ class GradientProblem:
         mp = mats[:, perm][:, :, perm]
         dp = dconst[:, perm][:, :, perm]
         torch = _torch()
-        if prop.n > _lib.MAX_N:
+        if not derivatives:
+            # forward only: the adjoint's tables (whose pattern may exceed
+            # the adjoint kernels' limits where the forward's does not) are
+            # not built; backward refuses such a state
+            tables = dict(d_col=None, d_val_f=None, d_col_t=None,
+                          d_val_t=None, d_dval_t=None, slots=prop.ell.slots,
+                          slots_t=0, slots_f=0)
+        elif prop.n > _lib.MAX_N:
             if grad_gen >= 0:
                 raise NotImplementedError(
                     "amplitude gradients need n <= 64")
@@ -407,6 +414,10 @@ class GradientProblem:
         record) and optionally ``lambda`` (n_inst, n), dLoss/d rho0 in the
         standard basis.
         """
+        if not state.get("derivatives", True):
+            raise ValueError("this state was set up without derivatives "
+                             "(setup(..., derivatives=False) or readout); "
+                             "use forward(theta, amplitudes) for gradients")
         torch = _torch()
         prop = state["prop"]
         n_inst = state["n_inst"]

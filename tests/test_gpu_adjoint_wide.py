@@ -4,7 +4,10 @@ The 21-level NV + 14N model (n = 171 compressed, n = 441 coherent) with
 gradients of the integrated fluorescence with respect to the 7-level
 parameters and the 14N quadrupole / hyperfine constants (12 in all).  A
 20 MHz drive and a random block-diagonal initial state (coherences in every
-orbital block) make every parameter enter at first order.
+orbital block) make every parameter but T2* enter at first order.  T2*
+enters no generator of this model: the reference's label combination drops
+the dephasing of an atom with both an electron and a nuclear spin
+(gg.py:2226-2314), so its gradient is exactly 0.
 
 * the gradient oracle (oracle/adjoint.py: dense scipy expm and
   expm_frechet, pinned by tests/test_oracle_adjoint.py) on short runs
@@ -140,7 +143,8 @@ def test_wide_checkpoint_interval_does_not_change_gradients(longer):
     assert np.array_equal(res["lambda"], res2["lambda"])
 
 
-FD_PARAMS = ("D_gs", "Bz", "radiative", "quadrupole", "A_par", "A_perp")
+#: every parameter of the 21-level gradient
+FD_PARAMS = configs.THETA21_NAMES
 
 
 def test_wide_theta_gradient_central_fd(longer):
@@ -166,3 +170,6 @@ def test_wide_theta_gradient_central_fd(longer):
         print("%-10s adjoint %+.10e  fd %+.10e  err/bound %.2f"
               % (name, adj[0], fd[0], np.max(np.abs(adj - fd) / bound)))
         assert np.all(np.abs(adj - fd) <= bound), name
+        if name == "T2star":
+            # no generator depends on it: both sides are exactly zero
+            assert np.all(adj == 0.0) and np.all(fd == 0.0)
