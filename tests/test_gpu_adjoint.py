@@ -318,3 +318,13 @@ def test_handoff_overrun_poisons_every_gradient(gpu, monkeypatch, store):
     # the bound restored: finite again
     r = pb.backward(state, np.ones((COUNT, 1)))
     assert np.all(np.isfinite(r["theta"].cpu().numpy()))
+
+
+def test_backward_refuses_forward_only_state(workload):
+    """A state set up without derivatives (as readout builds it) has no
+    adjoint tables: backward says so instead of launching."""
+    pb = workload.problem
+    state = pb.setup(workload.theta, workload.amplitudes, derivatives=False)
+    assert state["d_col"] is None and not state["derivatives"]
+    with pytest.raises(ValueError, match="without derivatives"):
+        pb.backward(state, np.ones((COUNT, 1)))

@@ -2,8 +2,9 @@
 
 Each instance's waveform is evaluated on the device; the oracle runs the
 reference algorithm (C restatement) with the same waveform as a Python
-sampler.  Bar: 1e-10 relative to the trace's max |value| (BASELINE
-definition of relative error for populations and fluorescence).
+sampler.  Bar: 1e-10 relative to the trace's max |value| (relative error
+of a population or fluorescence trace is taken per trace, against its
+largest magnitude).
 """
 
 import numpy as np
