@@ -92,6 +92,36 @@ def test_wide_states_vs_oracle(gpu, use_incoherent, count):
         assert err_obs <= TOL
 
 
+@pytest.mark.parametrize("use_incoherent,n_out", [(True, 400),
+                                                   (False, 20)])
+def test_wide_longer_run_vs_c_oracle(gpu, use_incoherent, n_out):
+    """One sweep point over 1600 steps (n = 171) and 80 steps (n = 441)
+    against the reference algorithm in C (70 dense squarings of the
+    171- or 441-dimensional exponentials per step): every state, 1e-10
+    relative to the largest entry."""
+    from oracle.waves import vectorised_sampler
+    wl = _short_workload(1, use_incoherent)
+    waves = wl.waveforms()
+    sim = wl.simulator()
+    t_end = n_out * wl.time_step * (1 + 1e-12)
+    print(f"{wl.name}: {n_out * wl.fine} steps on the GPU", flush=True)
+    full = sim.run(waves, wl.rho0, 0.0, t_end, wl.time_step,
+                   read="full").cpu().numpy()[0]
+    print("  oracle (cpu_ref.c)", flush=True)
+    osim = oracle_gen(None, wl.generators, wl.vmap, backend="c",
+                      number_of_exponentials=wl.number_of_exponentials,
+                      number_of_fine_divisions=wl.fine,
+                      vectorised_sampler=vectorised_sampler(waves[0]))
+    _, dens = osim(wl.rho0_matrix, 0.0, t_end, wl.time_step)
+    ref = np.stack([np.array([
+        (d[y, x].real if c == 0 else d[y, x].imag)
+        for (y, x, c) in wl.vmap]) for d in dens])
+    assert full.shape == ref.shape == (n_out, wl.n)
+    err = np.max(np.abs(full - ref)) / np.max(np.abs(ref))
+    print(f"  n={wl.n}: state rel err {err:.3e}", flush=True)
+    assert err <= TOL
+
+
 def test_wide_vectorisations_agree_full_length(gpu):
     """250,000 steps at four sweep points, n = 441 vs n = 171."""
     freqs = np.array([2.8656e9, 2.8678e9, 2.8700e9, 2.8722e9])
