@@ -406,14 +406,22 @@ def secondary_leg(name):
     wall = time.perf_counter() - t0
     out = run.result()
     steps = n_inst * wl.steps_per_instance()
-    flops = sim.step_flops(plan) * steps / (ev[0].elapsed_time(ev[1]) / 1e3)
+    seconds = ev[0].elapsed_time(ev[1]) / 1e3
+    flops = sim.step_flops(plan) * steps / seconds
     extra = {}
     if chunked:
+        # the chunk propagators advance n basis vectors through every
+        # chunk's L outputs: n times the one-chain work, plus the combine
+        executed = (n_inst * plan["chunks"] * wl.n * plan["chunk_out"] *
+                    wl.fine)
         extra = {"time_chunks": plan["chunks"],
                  "outputs_per_chunk": plan["chunk_out"],
                  "basis_vectors_per_wavefront": plan["chunk_pack"],
                  "launch": "chunk propagators (ssmx_propagate, n basis "
-                           "vectors per chunk) + ssmx_chunk_combine"}
+                           "vectors per chunk) + ssmx_chunk_combine",
+                 "executed_member_steps": executed,
+                 "executed_tflops":
+                     sim.step_flops(plan) * executed / seconds / 1e12}
     return {"workload": wl.name, "instances": n_inst,
             "superoperator_n": wl.n,
             "steps_per_instance": wl.steps_per_instance(),
