@@ -133,7 +133,12 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
 // columns.  NPAIR: leading slot pairs fetched whole (ell_apply).  MG > 0:
 // the MG (= M) generator rows are read from LDS once into registers;
 // MG < 0: the first -MG rows are, the others are read at each exponent.
-template <int SLOTS, int NPAIR, int TAY, int MG = 0, bool STORE = false>
+// UNIT0 (MG = 2, two exponentials, lab frame, waveform mode): matrix 0's
+// coefficient is 1 at every node (the folded constant generators), so its
+// share of each exponent, fma(f0_j, G0, 0), is formed once per launch; f0_j
+// is computed as step_factor computes it, so the rows are the same bits.
+template <int SLOTS, int NPAIR, int TAY, int MG = 0, bool STORE = false,
+          bool UNIT0 = false>
 __device__ __forceinline__ void propagate_instance(const Params &p,
                                                    const double *vals,
                                                    double *xbuf, int M,
@@ -161,6 +166,22 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                 g[m][sl] = vals[((m * (SLOTP / 2) + sl / 2) * kWave + lane) *
                                     2 +
                                 (sl & 1)];
+    }
+    static_assert(!UNIT0 || (MG == 2 && !STORE), "UNIT0 needs MG = 2");
+    double cst[UNIT0 ? 2 : 1][SLOTS];
+    if constexpr (UNIT0) {
+        const int S = d.n_node;
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            // step_factor's h sum_s w[j,s] c(t_s) with c = 1
+            double cw = 0.0;
+            for (int s = 0; s < S; ++s)
+                cw = fma(p.w[jj * S + s], 1.0, cw);
+            const double f0 = d.h * cw;
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; ++sl)
+                cst[jj][sl] = fma(f0, g[0][sl], 0.0);
+        }
     }
 
     // Per-instance views.
@@ -237,15 +258,8 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                              : 0.0;
             }
             const double x_prev = x;
-            for (int j = J - 1; j >= 0; --j) {
-                double a[SLOTS];
-                if constexpr (MG > 0)
-                    assemble_row_reg<SLOTS, MG>(a, g, factor, (b * J + j) * M);
-                else if constexpr (MG < 0)
-                    assemble_row_mix<SLOTS, MGR>(a, g, vals, factor,
-                                                 (b * J + j) * M, M, lane);
-                else
-                    assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
+            // exponential j with its row a assembled
+            auto exponential = [&](int j, double (&a)[SLOTS]) {
                 if (d.substeps > 1) {
                     keep_branch();
 #pragma unroll
@@ -264,6 +278,32 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                     x = taylor_action<SLOTS, NPAIR, TAY>(
                         a, col, x, xbuf, lane, use_partner, d.taylor,
                         inv_lane);
+            };
+            if constexpr (UNIT0) {
+#pragma unroll
+                for (int jj = 1; jj >= 0; --jj) {
+                    double a[SLOTS];
+                    const double f = read_lane(factor, (b * 2 + jj) * 2 + 1);
+#pragma unroll
+                    for (int sl = 0; sl < SLOTS; ++sl)
+                        a[sl] = fma(f, g[1][sl], cst[jj][sl]);
+                    exponential(jj, a);
+                }
+            } else {
+                for (int j = J - 1; j >= 0; --j) {
+                    double a[SLOTS];
+                    if constexpr (MG > 0)
+                        assemble_row_reg<SLOTS, MG>(a, g, factor,
+                                                    (b * J + j) * M);
+                    else if constexpr (MG < 0)
+                        assemble_row_mix<SLOTS, MGR>(a, g, vals, factor,
+                                                     (b * J + j) * M, M,
+                                                     lane);
+                    else
+                        assemble_row(a, vals, factor, (b * J + j) * M, M,
+                                     lane);
+                    exponential(j, a);
+                }
             }
             if (rotating) {
                 const double xp = swap_pair(x);
@@ -308,12 +348,13 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
 // per-term exits for degrees up to kUnrolledMax (SLOTS <= 16), else the
 // runtime loop.  Separate kernels keep each chain's register allocation
 // apart (both in one kernel ran every degree 1-2 % slower).
-template <int SLOTS, int NPAIR, int CHAIN, int MG, bool STORE = false>
+template <int SLOTS, int NPAIR, int CHAIN, int MG, bool STORE = false,
+          bool UNIT0 = false>
 __device__ __forceinline__ void propagate_instance_any(const Params &p, const double *vals, double *xbuf, int M, int inst, const int (&col)[SLOTS])
 {
     if constexpr (CHAIN > 0) {
-        propagate_instance<SLOTS, NPAIR, CHAIN, MG, STORE>(p, vals, xbuf, M,
-                                                          inst, col);
+        propagate_instance<SLOTS, NPAIR, CHAIN, MG, STORE, UNIT0>(
+            p, vals, xbuf, M, inst, col);
     } else {
         if constexpr (SLOTS <= 16) {
             if (p.d.taylor >= 1 && p.d.taylor <= kUnrolledMax) {
@@ -323,6 +364,35 @@ __device__ __forceinline__ void propagate_instance_any(const Params &p, const do
         }
         propagate_instance<SLOTS, NPAIR, 0>(p, vals, xbuf, M, inst, col);
     }
+}
+
+// The instance's layout variant: slot pairs read whole (every one, the
+// first, or none; other counts take the largest variant below them, which
+// reads the same values).
+template <int SLOTS, int CHAIN, int MG, bool STORE, bool UNIT0>
+__device__ __forceinline__ void dispatch_layout(const Params &p,
+                                                const double *lds,
+                                                double *xbuf, int M, int inst,
+                                                const int (&col)[SLOTS],
+                                                int np)
+{
+    if constexpr (SLOTS >= 4) {
+        constexpr int kAll = (SLOTS - 2) / 2;
+        if constexpr (kAll > 1 && SLOTS <= 16) {
+            if (np >= kAll) {
+                propagate_instance_any<SLOTS, kAll, CHAIN, MG, STORE, UNIT0>(
+                    p, lds, xbuf, M, inst, col);
+                return;
+            }
+        }
+        if (np >= 1) {
+            propagate_instance_any<SLOTS, 1, CHAIN, MG, STORE, UNIT0>(
+                p, lds, xbuf, M, inst, col);
+            return;
+        }
+    }
+    propagate_instance_any<SLOTS, 0, CHAIN, MG, STORE, UNIT0>(p, lds, xbuf,
+                                                              M, inst, col);
 }
 
 template <int SLOTS, int CHAIN, int MG, bool STORE = false>
@@ -377,20 +447,19 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
     // (the other pair of a row's quad) or none; other counts take the
     // largest variant below them, which reads the same values
     const int np = fetched_pairs(col, lane, d.partner != 0);
-    if constexpr (SLOTS >= 4) {
-        constexpr int kAll = (SLOTS - 2) / 2;
-        if constexpr (kAll > 1 && SLOTS <= 16) {
-            if (np >= kAll) {
-                propagate_instance_any<SLOTS, kAll, CHAIN, MG, STORE>(p, lds, xbuf, M, inst, col);
-                return;
-            }
-        }
-        if (np >= 1) {
-            propagate_instance_any<SLOTS, 1, CHAIN, MG, STORE>(p, lds, xbuf, M, inst, col);
+    if constexpr (MG == 2 && CHAIN > 0 && !STORE) {
+        // matrix 0 with coefficient 1 at every node: the instance's
+        // waveform gives generator 0 the base 1 and no segments (the
+        // folded constants); wave-uniform, read once
+        if (!rotating && (d.flags & SSMX_COEF_WAVE) && d.n_exp == 2 &&
+            p.waves[inst].seg_count[0] == 0 && p.waves[inst].base[0] == 1.0) {
+            dispatch_layout<SLOTS, CHAIN, MG, STORE, true>(p, lds, xbuf, M,
+                                                           inst, col, np);
             return;
         }
     }
-    propagate_instance_any<SLOTS, 0, CHAIN, MG, STORE>(p, lds, xbuf, M, inst, col);
+    dispatch_layout<SLOTS, CHAIN, MG, STORE, false>(p, lds, xbuf, M, inst,
+                                                    col, np);
 }
 
 template <int SLOTS, int CHAIN, int MG, bool STORE = false>
