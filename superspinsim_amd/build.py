@@ -18,9 +18,9 @@ SOURCES = ["ssmx_propagate.hip", "ssmx_adjoint.hip", "ssmx_queue.hip",
            "ssmx_wide.hip", "ssmx_adjoint_wide.hip", "ssmx_dense.hip",
            "ssmx_chunks.hip"]
 
-# -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32 (see DESIGN notes on
-# packed FP32 on this hardware); the host pass warns that it does not know
-# the feature, which is harmless.
+# -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32 (some of their forms
+# have given wrong results on MI355X without any error); the host pass warns
+# that it does not know the feature, which is harmless.
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -59,16 +59,10 @@ constexpr int kRing = 3;
 // from the host so that every kernel still exits normally).
 constexpr long kMaxPolls = 10000000;
 
-// 1/q, q = 0..40 (entry 0 unused); the recompute uses the same table and
-// recursion as the propagator so that it reproduces the forward states.
-__constant__ double kInvA[SSMX_MAX_TAYLOR + 1] = {
-    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
-    1.0 / 6,    1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
-    1.0 / 12,   1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
-    1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
-    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
-    1.0 / 30,   1.0 / 31,   1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35,
-    1.0 / 36,   1.0 / 37,   1.0 / 38, 1.0 / 39, 1.0 / 40};
+// 1 / q!, q = 0..40; the recompute uses the same table and recursion as
+// the propagator so that it reproduces the forward states.
+__constant__ ssmx::InvFactorials kInvA =
+    ssmx::make_inv_factorials();
 
 struct AdjParams {
     ssmx_adjoint_desc d;
@@ -171,7 +165,11 @@ __device__ __forceinline__ double ell_product(const double (&a)[SL],
         else
             y0 = fma(a[sl], g[sl], y0);
     }
-    return y0 + y1;
+    {
+        // one rounding, never contracted with a[0] * g[0] (SL = 2)
+#pragma clang fp contract(off)
+        return y0 + y1;
+    }
 }
 
 // a[sl] = sum_m f_m V_m[sl] with f_m broadcast from lane base + m of
@@ -318,7 +316,7 @@ adjoint_kernel(AdjParams p)
 #pragma unroll
         for (int sl = 1; sl < SF; ++sl)
             cf[sl] = sl < d.slots ? p.col_f[sl * kWave + lane] : lane;
-        const double inv_lane = kInvA[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+        const double inv_lane = kInvA.v[lane <= SSMX_MAX_TAYLOR ? lane : 0];
         const int JM = J * M;
         const int B = kWave / JM;          // steps per coefficient batch
         const int my_b = lane / JM;
@@ -588,33 +586,30 @@ adjoint_kernel(AdjParams p)
                         }
                     }
                     {
-                        // the propagator's roundings (no contraction)
+                        // the propagator's roundings: y_t = A y_(t-1)
+                        // unscaled, acc += y_t / t! in one fma
 #pragma clang fp contract(off)
-                        term = (y0 + y1) * (1.0 / t);
+                        term = y0 + y1;
                         if (t <= m_fwd)
-                            acc += term;
+                            acc = fma(term, ssmx::inv_factorial(t), acc);
                     }
                     W[t] = (v0 + v1) * (1.0 / t);
                 }
                 // the rest of the recomputation's chain: for the common
-                // degree with constant 1 / t (a v_readlane result would
+                // degree with constant 1 / t! (a v_readlane result would
                 // stall each product), else the loop
                 if (m_fwd == kUnrolledTaylor) {
 #pragma unroll
                     for (int t = MA; t <= kUnrolledTaylor; ++t) {
-#pragma clang fp contract(off)
                         term = ell_product<SF>(aR, cf, term, x0, lane,
-                                               use_partner) *
-                               (1.0 / t);
-                        acc += term;
+                                               use_partner);
+                        acc = fma(term, ssmx::inv_factorial(t), acc);
                     }
                 } else {
                     for (int t = MA; t <= m_fwd; ++t) {
-#pragma clang fp contract(off)
                         term = ell_product<SF>(aR, cf, term, x0, lane,
-                                               use_partner) *
-                               read_lane(inv_lane, t);
-                        acc += term;
+                                               use_partner);
+                        acc = fma(term, read_lane(inv_lane, t), acc);
                     }
                 }
                 if (r_on) {

@@ -176,7 +176,12 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
         else
             y0 = fma(a[sl], g[sl], y0);
     }
-    return y0 + y1;
+    {
+        // one rounding, never contracted with a[0] * g[0] (SLOTS = 2), so
+        // every kernel that repeats this product gets the same bits
+#pragma clang fp contract(off)
+        return y0 + y1;
+    }
 }
 
 // In a wave-uniform branch: keeps it a branch.  Without it the compiler
@@ -278,6 +283,30 @@ __device__ __forceinline__ void assemble_row_mix(double (&a)[SLOTS],
 // scalar moves at the point of use: as plain constants the compiler hoists
 // all of them out of the step loop, where they spill and come back through
 // v_readlane, whose SGPR result stalls the next multiply.
+// 1 / t! in double, the same value wherever it is formed: the product
+// 2 * 3 * ... * t rounded in that order (exact up to t = 18), then 1 / it.
+__host__ __device__ constexpr double inv_factorial(int t)
+{
+    double f = 1.0;
+    for (int k = 2; k <= t; ++k)
+        f *= (double)k;
+    return 1.0 / f;
+}
+
+// inv_factorial(0 .. SSMX_MAX_TAYLOR), for per-lane tables of the Taylor
+// loop (lane t holds 1 / t!).
+struct InvFactorials {
+    double v[SSMX_MAX_TAYLOR + 1];
+};
+
+__host__ __device__ constexpr InvFactorials make_inv_factorials()
+{
+    InvFactorials r{};
+    for (int t = 0; t <= SSMX_MAX_TAYLOR; ++t)
+        r.v[t] = inv_factorial(t);
+    return r;
+}
+
 template <int T>
 __device__ __forceinline__ double scalar_inverse()
 {
@@ -288,8 +317,19 @@ __device__ __forceinline__ double scalar_inverse()
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// inv_factorial(T) built where it is used, as scalar_inverse builds 1 / T.
+template <int T>
+__device__ __forceinline__ double scalar_inv_factorial()
+{
+    constexpr uint64_t bits = __builtin_bit_cast(uint64_t, inv_factorial(T));
+    uint32_t lo, hi;
+    asm volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"((uint32_t)bits));
+    asm volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"((uint32_t)(bits >> 32)));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Terms T..M of taylor_action's unrolled chain, stopping after term m
-// (template recursion, so every 1.0 / T is a compile-time constant).
+// (template recursion, so every 1 / T! is a compile-time constant).
 template <int SLOTS, int NPAIR, int M, int T = 1>
 __device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
                                              const int (&col)[SLOTS],
@@ -300,35 +340,32 @@ __device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
     if constexpr (T <= M) {
         if (T > m)
             return;
-        // 1 and 1/2 stay plain constants (the multiply by 1 folds away, 0.5
-        // is an inline operand)
+        // 1 / 1! and 1 / 2! stay plain constants (inline operands)
         double inv;
         if constexpr (T <= 2)
-            inv = 1.0 / T;
+            inv = inv_factorial(T);
         else
-            inv = scalar_inverse<T>();
-        {
-#pragma clang fp contract(off)
-            term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
-                                           partner) *
-                   inv;
-            acc += term;
-        }
+            inv = scalar_inv_factorial<T>();
+        term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane, partner);
+        acc = fma(term, inv, acc);
         taylor_chain<SLOTS, NPAIR, M, T + 1>(a, col, term, acc, xbuf, lane,
                                              partner, m);
     }
 }
 
-// exp(A) x by its Taylor series of degree m: term <- (A term) * (1 / t),
-// acc += term for t = 1..m, each product and sum rounded on its own (no
-// contraction), so every kernel that repeats the chain gets the same bits.
-// M > 0: the degree is M, the chain is unrolled and the 1 / t are plain
-// constants.  M < 0: the chain is unrolled up to -M terms and stops after
-// the runtime degree m <= -M (one scalar branch per term), each 1 / t built
-// at its use (scalar_inverse).  M = 0: a loop over the runtime degree m,
-// 1 / t broadcast from lane t of inv_lane (which holds 1 / lane).  All
-// three give the same doubles in the same order; the unrolled forms avoid
-// the loop's v_readlane, whose SGPR result stalls the dependent chain.
+// exp(A) x by its Taylor series of degree m: y_t = A y_(t-1) from y_0 = x
+// (the unscaled powers, each product rounded as ell_apply rounds it) and
+// acc <- fma(y_t, 1 / t!, acc) for t = 1..m, one rounding per term, so
+// every kernel that repeats the chain gets the same bits.  Carrying the
+// powers unscaled saves the multiply of each term by 1 / t on the VALU and
+// on the chain's critical path.  M > 0: the degree is M, the chain is
+// unrolled and the 1 / t! are plain constants.  M < 0: the chain is unrolled
+// up to -M terms and stops after the runtime degree m <= -M (one scalar
+// branch per term), each 1 / t! built at its use (scalar_inv_factorial).
+// M = 0: a loop over the runtime degree m, 1 / t! broadcast from lane t of
+// inv_lane (which holds 1 / lane!).  All three give the same doubles in
+// the same order; the unrolled forms avoid the loop's v_readlane, whose
+// SGPR result stalls the dependent chain.
 template <int SLOTS, int NPAIR, int M>
 __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
                                                 const int (&col)[SLOTS],
@@ -341,11 +378,9 @@ __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
     if constexpr (M > 0) {
 #pragma unroll
         for (int t = 1; t <= M; ++t) {
-#pragma clang fp contract(off)
             term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
-                                           partner) *
-                   (1.0 / t);
-            acc += term;
+                                           partner);
+            acc = fma(term, inv_factorial(t), acc);
         }
     } else if constexpr (M < 0) {
         // The degree through an opaque scalar move: the per-term exit tests
@@ -358,11 +393,9 @@ __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
                                        partner, mm);
     } else {
         for (int t = 1; t <= m; ++t) {
-#pragma clang fp contract(off)
             term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
-                                           partner) *
-                   read_lane(inv_lane, t);
-            acc += term;
+                                           partner);
+            acc = fma(term, read_lane(inv_lane, t), acc);
         }
     }
     return acc;

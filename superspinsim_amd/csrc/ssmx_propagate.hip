@@ -62,15 +62,9 @@ using ssmx::wave_sum;
 
 constexpr int kWavesPerBlock = 8;
 
-// 1/q for the Taylor recursion, q = 0..40 (entry 0 unused).
-__constant__ double kInv[SSMX_MAX_TAYLOR + 1] = {
-    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
-    1.0 / 6,    1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
-    1.0 / 12,   1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
-    1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
-    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
-    1.0 / 30,   1.0 / 31,   1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35,
-    1.0 / 36,   1.0 / 37,   1.0 / 38, 1.0 / 39, 1.0 / 40};
+// 1 / q! for the Taylor loop, q = 0..40 (lane q reads entry q).
+__constant__ ssmx::InvFactorials kInv =
+    ssmx::make_inv_factorials();
 
 struct Params {
     ssmx_propagate_desc d;
@@ -194,7 +188,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
     // Lane t holds 1/t: the Taylor recursion broadcasts it with readlane
     // (a scalar memory load or a division would sit in the dependent
     // chain of every term).
-    const double inv_lane = kInv[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+    const double inv_lane = kInv.v[lane <= SSMX_MAX_TAYLOR ? lane : 0];
 
     double x = lane < n ? p.rho0[(size_t)inst * n + lane] : 0.0;
 

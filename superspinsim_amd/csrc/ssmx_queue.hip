@@ -39,14 +39,9 @@ using ssmx::wave_sum;
 
 constexpr int kMaxWavesPerBlock = 8;
 
-__constant__ double kInvQ[SSMX_MAX_TAYLOR + 1] = {
-    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
-    1.0 / 6,    1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
-    1.0 / 12,   1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
-    1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
-    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
-    1.0 / 30,   1.0 / 31,   1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35,
-    1.0 / 36,   1.0 / 37,   1.0 / 38, 1.0 / 39, 1.0 / 40};
+// 1 / q! for the Taylor loop, q = 0..40 (lane q reads entry q).
+__constant__ ssmx::InvFactorials kInvQ =
+    ssmx::make_inv_factorials();
 
 struct QParams {
     ssmx_propagate_desc d;
@@ -108,7 +103,7 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
     const int n = d.n;
     const int J = d.n_exp;
     const bool use_partner = d.partner != 0;
-    const double inv_lane = kInvQ[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+    const double inv_lane = kInvQ.v[lane <= SSMX_MAX_TAYLOR ? lane : 0];
     double g[MG > 0 ? MG : 1][SLOTS];
     if constexpr (MG > 0) {
         constexpr int SLOTP = SLOTS + (SLOTS & 1);
