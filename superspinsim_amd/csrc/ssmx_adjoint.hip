@@ -156,20 +156,11 @@ __device__ __forceinline__ double ell_product(const double (&a)[SL],
     for (int sl = 2; sl < SL; ++sl)
         g[sl] = xbuf[col[sl]];
     wave_exchange();
-    double y0 = a[0] * g[0];
-    double y1 = 0.0;
+    double y = a[0] * g[0];
 #pragma unroll
-    for (int sl = 1; sl < SL; ++sl) {
-        if (sl & 1)
-            y1 = fma(a[sl], g[sl], y1);
-        else
-            y0 = fma(a[sl], g[sl], y0);
-    }
-    {
-        // one rounding, never contracted with a[0] * g[0] (SL = 2)
-#pragma clang fp contract(off)
-        return y0 + y1;
-    }
+    for (int sl = 1; sl < SL; ++sl)
+        y = fma(a[sl], g[sl], y);
+    return y;
 }
 
 // a[sl] = sum_m f_m V_m[sl] with f_m broadcast from lane base + m of
@@ -570,30 +561,20 @@ adjoint_kernel(AdjParams p)
                         gV[sl] = g.y;
                     }
                     ssmx::wave_exchange();
-                    // the propagator's product: same terms, same order
-                    double y0 = aR[0] * gR[0];
-                    double y1 = 0.0;
-                    double v0 = aV[0] * gV[0];
-                    double v1 = 0.0;
+                    // the propagator's product and roundings: same terms,
+                    // same order; y_t = A y_(t-1) unscaled, acc += y_t / t!
+                    // in one fma
+                    double y = aR[0] * gR[0];
+                    double v = aV[0] * gV[0];
 #pragma unroll
                     for (int sl = 1; sl < SF; ++sl) {
-                        if (sl & 1) {
-                            y1 = fma(aR[sl], gR[sl], y1);
-                            v1 = fma(aV[sl], gV[sl], v1);
-                        } else {
-                            y0 = fma(aR[sl], gR[sl], y0);
-                            v0 = fma(aV[sl], gV[sl], v0);
-                        }
+                        y = fma(aR[sl], gR[sl], y);
+                        v = fma(aV[sl], gV[sl], v);
                     }
-                    {
-                        // the propagator's roundings: y_t = A y_(t-1)
-                        // unscaled, acc += y_t / t! in one fma
-#pragma clang fp contract(off)
-                        term = y0 + y1;
-                        if (t <= m_fwd)
-                            acc = fma(term, ssmx::inv_factorial(t), acc);
-                    }
-                    W[t] = (v0 + v1) * (1.0 / t);
+                    term = y;
+                    if (t <= m_fwd)
+                        acc = fma(term, ssmx::inv_factorial(t), acc);
+                    W[t] = v * (1.0 / t);
                 }
                 // the rest of the recomputation's chain: for the common
                 // degree with constant 1 / t! (a v_readlane result would

@@ -130,9 +130,11 @@ __device__ __forceinline__ int fetched_pairs(const int (&col)[SLOTS],
 // are gathered one by one.  The LDS reads see `xbuf` (one
 // double per lane of the wavefront, 16-byte aligned) after every lane has
 // written its component; LDS instructions of a wavefront execute in order,
-// so a wavefront-scope fence is the only barrier.  Even and odd slots are
-// summed in two chains in slot order, so the rounding depends on the row's
-// values and columns only, not on how the components were fetched.
+// so a wavefront-scope fence is the only barrier.  The slots are summed in
+// one fma chain in slot order (the register-held slots first, the LDS
+// gathers last, so one fma follows the last gather), and the rounding
+// depends on the row's values and columns only, not on how the components
+// were fetched.
 template <int SLOTS, int NPAIR>
 __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
                                             const int (&col)[SLOTS],
@@ -167,21 +169,11 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
     for (int sl = 2 + 2 * NPAIR; sl < SLOTS; ++sl)
         g[sl] = xbuf[col[sl]];
     wave_exchange();
-    double y0 = a[0] * g[0];
-    double y1 = 0.0;
+    double y = a[0] * g[0];
 #pragma unroll
-    for (int sl = 1; sl < SLOTS; ++sl) {
-        if (sl & 1)
-            y1 = fma(a[sl], g[sl], y1);
-        else
-            y0 = fma(a[sl], g[sl], y0);
-    }
-    {
-        // one rounding, never contracted with a[0] * g[0] (SLOTS = 2), so
-        // every kernel that repeats this product gets the same bits
-#pragma clang fp contract(off)
-        return y0 + y1;
-    }
+    for (int sl = 1; sl < SLOTS; ++sl)
+        y = fma(a[sl], g[sl], y);
+    return y;
 }
 
 // In a wave-uniform branch: keeps it a branch.  Without it the compiler
