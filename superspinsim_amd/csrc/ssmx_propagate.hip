@@ -122,6 +122,17 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
     return d.h * cw;
 }
 
+// step_factor's sum_s w[j,s] c(t_s) for a coefficient c = 1 at every node,
+// in step_factor's order and roundings.
+__device__ __forceinline__ double unit_weight_sum(const double *w, int j,
+                                                  int n_node)
+{
+    double cw = 0.0;
+    for (int s = 0; s < n_node; ++s)
+        cw = fma(w[j * n_node + s], 1.0, cw);
+    return cw;
+}
+
 // One instance on one wavefront, after the kernel has staged the generator
 // values (`vals`, [m][slot / 2][lane][slot % 2]) and loaded the lane's ELL
 // columns.  NPAIR: leading slot pairs fetched whole (ell_apply).  MG > 0:
@@ -130,7 +141,10 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
 // UNIT0 (MG = 2, two exponentials, lab frame, waveform mode): matrix 0's
 // coefficient is 1 at every node (the folded constant generators), so its
 // share of each exponent, fma(f0_j, G0, 0), is formed once per launch; f0_j
-// is computed as step_factor computes it, so the rows are the same bits.
+// is computed as step_factor computes it, so the row is the same bits.  The
+// caller checks that f0_0 == f0_1 (CF4:2's two weight rows hold the same
+// two numbers, and their sums are equal), so one row serves both
+// exponentials.
 template <int SLOTS, int NPAIR, int TAY, int MG = 0, bool STORE = false,
           bool UNIT0 = false>
 __device__ __forceinline__ void propagate_instance(const Params &p,
@@ -141,7 +155,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
 {
     const ssmx_propagate_desc &d = p.d;
     const int lane = threadIdx.x & (kWave - 1);
-    const bool rotating = (d.flags & SSMX_ROTATING) != 0;
+    const bool rotating = !UNIT0 && (d.flags & SSMX_ROTATING) != 0;
     const int n = d.n;
     const int J = d.n_exp;
     const bool wave_mode = (d.flags & SSMX_COEF_WAVE) != 0;
@@ -162,20 +176,12 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                                 (sl & 1)];
     }
     static_assert(!UNIT0 || (MG == 2 && !STORE), "UNIT0 needs MG = 2");
-    double cst[UNIT0 ? 2 : 1][SLOTS];
+    double cst[SLOTS];
     if constexpr (UNIT0) {
-        const int S = d.n_node;
+        const double f0 = d.h * unit_weight_sum(p.w, 0, d.n_node);
 #pragma unroll
-        for (int jj = 0; jj < 2; ++jj) {
-            // step_factor's h sum_s w[j,s] c(t_s) with c = 1
-            double cw = 0.0;
-            for (int s = 0; s < S; ++s)
-                cw = fma(p.w[jj * S + s], 1.0, cw);
-            const double f0 = d.h * cw;
-#pragma unroll
-            for (int sl = 0; sl < SLOTS; ++sl)
-                cst[jj][sl] = fma(f0, g[0][sl], 0.0);
-        }
+        for (int sl = 0; sl < SLOTS; ++sl)
+            cst[sl] = fma(f0, g[0][sl], 0.0);
     }
 
     // Per-instance views.
@@ -280,7 +286,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                     const double f = read_lane(factor, (b * 2 + jj) * 2 + 1);
 #pragma unroll
                     for (int sl = 0; sl < SLOTS; ++sl)
-                        a[sl] = fma(f, g[1][sl], cst[jj][sl]);
+                        a[sl] = fma(f, g[1][sl], cst[sl]);
                     exponential(jj, a);
                 }
             } else {
@@ -446,7 +452,9 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         // waveform gives generator 0 the base 1 and no segments (the
         // folded constants); wave-uniform, read once
         if (!rotating && (d.flags & SSMX_COEF_WAVE) && d.n_exp == 2 &&
-            p.waves[inst].seg_count[0] == 0 && p.waves[inst].base[0] == 1.0) {
+            p.waves[inst].seg_count[0] == 0 && p.waves[inst].base[0] == 1.0 &&
+            unit_weight_sum(p.w, 0, d.n_node) ==
+                unit_weight_sum(p.w, 1, d.n_node)) {
             dispatch_layout<SLOTS, CHAIN, MG, STORE, true>(p, lds, xbuf, M,
                                                            inst, col, np);
             return;
