@@ -61,6 +61,8 @@ using ssmx::wave_coef;
 using ssmx::wave_sum;
 
 constexpr int kWavesPerBlock = 8;
+// readout rows staged in LDS (obs readouts hold 1..2)
+constexpr int kObsRows = 2;
 
 // 1 / q! for the Taylor loop, q = 0..40 (lane q reads entry q).
 __constant__ ssmx::InvFactorials kInv =
@@ -122,6 +124,26 @@ __device__ double step_factor(const Params &p, long step, int j, int m,
     return d.h * cw;
 }
 
+// The lane's ELL columns.  Slot 0 is the diagonal by definition
+// (ell_col[0][row] holds the row's used-slot count); slot 1 is the partner
+// r ^ 1 when `partner` is set; other slots hold column indices.  Unused
+// slots point at the row itself with value 0, so every slot is read without
+// predication.  The empty asm with a memory clobber keeps each call's loads
+// its own: the kernel reads the columns to choose the layout variant, the
+// variant again for its loops, and no copy stays live (or spilled) across
+// the choice.
+template <int SLOTS>
+__device__ __forceinline__ void load_columns(const Params &p, int lane,
+                                             int (&col)[SLOTS])
+{
+    asm volatile("" ::: "memory");
+    col[0] = lane;
+#pragma unroll
+    for (int s = 1; s < SLOTS; ++s)
+        col[s] = s < p.d.slots ? p.ell_col[s * kWave + lane]
+                               : pad_column(s, lane);
+}
+
 // step_factor's sum_s w[j,s] c(t_s) for a coefficient c = 1 at every node,
 // in step_factor's order and roundings.
 __device__ __forceinline__ double unit_weight_sum(const double *w, int j,
@@ -150,11 +172,14 @@ template <int SLOTS, int NPAIR, int TAY, int MG = 0, bool STORE = false,
 __device__ __forceinline__ void propagate_instance(const Params &p,
                                                    const double *vals,
                                                    double *xbuf, int M,
-                                                   int inst,
-                                                   const int (&col)[SLOTS])
+                                                   int inst)
 {
     const ssmx_propagate_desc &d = p.d;
     const int lane = threadIdx.x & (kWave - 1);
+    // the columns read again here, so that the kernel's copy (read to
+    // choose the layout variant) is dead before the loops
+    int col[SLOTS];
+    load_columns<SLOTS>(p, lane, col);
     const bool rotating = !UNIT0 && (d.flags & SSMX_ROTATING) != 0;
     const int n = d.n;
     const int J = d.n_exp;
@@ -203,16 +228,21 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
         eig_own = p.eig[lane];
         eig_part = p.eig[kWave + lane];
     }
-    double obs0 = 0.0, obs1 = 0.0, off0 = 0.0, off1 = 0.0;
-    if (!read_full) {
-        obs0 = p.obs[lane];
-        off0 = p.obs_offset ? p.obs_offset[(size_t)inst * d.n_obs] : 0.0;
-        if (d.n_obs > 1) {
-            obs1 = p.obs[kWave + lane];
-            off1 = p.obs_offset ? p.obs_offset[(size_t)inst * d.n_obs + 1]
-                                : 0.0;
-        }
+    // observable k on state v: its row is read from LDS at each readout
+    // (once per output interval; held in registers across the step loop
+    // the rows would spill), its wave-uniform offset once
+    constexpr int SLOTP = SLOTS + (SLOTS & 1);
+    const double *obs_rows = vals + M * SLOTP * kWave + kWavesPerBlock * kWave;
+    double off0 = 0.0, off1 = 0.0;
+    if (!read_full && p.obs_offset) {
+        off0 = p.obs_offset[(size_t)inst * d.n_obs];
+        if (d.n_obs > 1)
+            off1 = p.obs_offset[(size_t)inst * d.n_obs + 1];
     }
+    auto observe = [&](int k, double v) {
+        return wave_sum(obs_rows[k * kWave + lane] * v) +
+               (k == 0 ? off0 : off1);
+    };
 
     // Lane-parallel coefficient batches: entry e = (b * J + j) * M + m.
     const int JM = J * M;
@@ -318,25 +348,24 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
             if (lane < n)
                 p.out[((size_t)inst * d.n_out + i) * n + lane] = x;
         } else if (!read_int) {
-            const double v0 = wave_sum(obs0 * x);
+            const double v0 = observe(0, x);
             if (lane == 0)
-                p.out[((size_t)inst * d.n_out + i) * d.n_obs] = v0 + off0;
+                p.out[((size_t)inst * d.n_out + i) * d.n_obs] = v0;
             if (d.n_obs > 1) {
-                const double v1 = wave_sum(obs1 * x);
+                const double v1 = observe(1, x);
                 if (lane == 0)
-                    p.out[((size_t)inst * d.n_out + i) * d.n_obs + 1] =
-                        v1 + off1;
+                    p.out[((size_t)inst * d.n_out + i) * d.n_obs + 1] = v1;
             }
         }
     }
     if (read_int) {
-        const double v0 = wave_sum(obs0 * xint);
+        const double v0 = observe(0, xint);
         if (lane == 0)
-            p.out[(size_t)inst * d.n_obs] = v0 + off0;
+            p.out[(size_t)inst * d.n_obs] = v0;
         if (d.n_obs > 1) {
-            const double v1 = wave_sum(obs1 * xint);
+            const double v1 = observe(1, xint);
             if (lane == 0)
-                p.out[(size_t)inst * d.n_obs + 1] = v1 + off1;
+                p.out[(size_t)inst * d.n_obs + 1] = v1;
         }
     }
     if (!STORE && p.state_out && ckpt_every == 0 && lane < n)
@@ -350,19 +379,19 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
 // apart (both in one kernel ran every degree 1-2 % slower).
 template <int SLOTS, int NPAIR, int CHAIN, int MG, bool STORE = false,
           bool UNIT0 = false>
-__device__ __forceinline__ void propagate_instance_any(const Params &p, const double *vals, double *xbuf, int M, int inst, const int (&col)[SLOTS])
+__device__ __forceinline__ void propagate_instance_any(const Params &p, const double *vals, double *xbuf, int M, int inst)
 {
     if constexpr (CHAIN > 0) {
         propagate_instance<SLOTS, NPAIR, CHAIN, MG, STORE, UNIT0>(
-            p, vals, xbuf, M, inst, col);
+            p, vals, xbuf, M, inst);
     } else {
         if constexpr (SLOTS <= 16) {
             if (p.d.taylor >= 1 && p.d.taylor <= kUnrolledMax) {
-                propagate_instance<SLOTS, NPAIR, -kUnrolledMax>(p, vals, xbuf, M, inst, col);
+                propagate_instance<SLOTS, NPAIR, -kUnrolledMax>(p, vals, xbuf, M, inst);
                 return;
             }
         }
-        propagate_instance<SLOTS, NPAIR, 0>(p, vals, xbuf, M, inst, col);
+        propagate_instance<SLOTS, NPAIR, 0>(p, vals, xbuf, M, inst);
     }
 }
 
@@ -373,7 +402,6 @@ template <int SLOTS, int CHAIN, int MG, bool STORE, bool UNIT0>
 __device__ __forceinline__ void dispatch_layout(const Params &p,
                                                 const double *lds,
                                                 double *xbuf, int M, int inst,
-                                                const int (&col)[SLOTS],
                                                 int np)
 {
     if constexpr (SLOTS >= 4) {
@@ -381,18 +409,18 @@ __device__ __forceinline__ void dispatch_layout(const Params &p,
         if constexpr (kAll > 1 && SLOTS <= 16) {
             if (np >= kAll) {
                 propagate_instance_any<SLOTS, kAll, CHAIN, MG, STORE, UNIT0>(
-                    p, lds, xbuf, M, inst, col);
+                    p, lds, xbuf, M, inst);
                 return;
             }
         }
         if (np >= 1) {
             propagate_instance_any<SLOTS, 1, CHAIN, MG, STORE, UNIT0>(
-                p, lds, xbuf, M, inst, col);
+                p, lds, xbuf, M, inst);
             return;
         }
     }
     propagate_instance_any<SLOTS, 0, CHAIN, MG, STORE, UNIT0>(p, lds, xbuf,
-                                                              M, inst, col);
+                                                              M, inst);
 }
 
 template <int SLOTS, int CHAIN, int MG, bool STORE = false>
@@ -403,8 +431,8 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
     Params p)
 {
     // LDS: generator values staged as [m][slot / 2][lane][slot % 2] (one
-    // ds_read_b128 returns two slots of a row), then one 64-double exchange
-    // buffer per wavefront.
+    // ds_read_b128 returns two slots of a row), one 64-double exchange
+    // buffer per wavefront, then the kObsRows readout rows.
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
     extern __shared__ double lds[];
     const ssmx_propagate_desc &d = p.d;
@@ -424,6 +452,11 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
                                      row]
                          : 0.0;
         }
+        if (!(d.flags & SSMX_READ_FULL)) {
+            double *obs_rows = lds + nval + kWavesPerBlock * kWave;
+            for (int e = threadIdx.x; e < d.n_obs * kWave; e += blockDim.x)
+                obs_rows[e] = p.obs[e];
+        }
         __syncthreads();
     }
 
@@ -433,20 +466,15 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         return;  // whole wavefront; no barrier follows
     double *xbuf = lds + M * SLOTP * kWave + (threadIdx.x / kWave) * kWave;
 
-    // Slot 0 is the diagonal by definition (ell_col[0][row] holds the row's
-    // used-slot count); slot 1 is the partner r ^ 1 when `partner` is set;
-    // other slots hold column indices.  Unused slots point at the row
-    // itself with value 0, so every slot is read without predication.
-    int col[SLOTS];
-    col[0] = lane;
-#pragma unroll
-    for (int s = 1; s < SLOTS; ++s)
-        col[s] = s < d.slots ? p.ell_col[s * kWave + lane]
-                             : pad_column(s, lane);
     // slot pairs read whole: every one (rows laid out by pairs), the first
     // (the other pair of a row's quad) or none; other counts take the
     // largest variant below them, which reads the same values
-    const int np = fetched_pairs(col, lane, d.partner != 0);
+    int np;
+    {
+        int col[SLOTS];
+        load_columns<SLOTS>(p, lane, col);
+        np = fetched_pairs(col, lane, d.partner != 0);
+    }
     if constexpr (MG == 2 && CHAIN > 0 && !STORE) {
         // matrix 0 with coefficient 1 at every node: the instance's
         // waveform gives generator 0 the base 1 and no segments (the
@@ -456,12 +484,12 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
             unit_weight_sum(p.w, 0, d.n_node) ==
                 unit_weight_sum(p.w, 1, d.n_node)) {
             dispatch_layout<SLOTS, CHAIN, MG, STORE, true>(p, lds, xbuf, M,
-                                                           inst, col, np);
+                                                           inst, np);
             return;
         }
     }
     dispatch_layout<SLOTS, CHAIN, MG, STORE, false>(p, lds, xbuf, M, inst,
-                                                    col, np);
+                                                    np);
 }
 
 template <int SLOTS, int CHAIN, int MG, bool STORE = false>
@@ -470,7 +498,7 @@ int launch_chain(const Params &p, hipStream_t stream)
     constexpr int SLOTP = SLOTS + (SLOTS & 1);
     const int M = ((p.d.flags & SSMX_ROTATING) ? p.d.n_node : 1) * p.d.n_gen;
     const size_t lds = ((size_t)M * SLOTP * kWave +
-                        (size_t)kWavesPerBlock * kWave) *
+                        (size_t)(kWavesPerBlock + kObsRows) * kWave) *
                        sizeof(double);
     if (lds > 160 * 1024)
         return set_error(SSMX_EINVAL,
