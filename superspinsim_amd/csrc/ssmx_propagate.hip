@@ -200,7 +200,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                                     2 +
                                 (sl & 1)];
     }
-    static_assert(!UNIT0 || (MG == 2 && !STORE), "UNIT0 needs MG = 2");
+    static_assert(!UNIT0 || MG == 2, "UNIT0 needs MG = 2");
     double cst[SLOTS];
     if constexpr (UNIT0) {
         const double f0 = d.h * unit_weight_sum(p.w, 0, d.n_node);
@@ -475,7 +475,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         load_columns<SLOTS>(p, lane, col);
         np = fetched_pairs(col, lane, d.partner != 0);
     }
-    if constexpr (MG == 2 && CHAIN > 0 && !STORE) {
+    if constexpr (MG == 2 && CHAIN > 0) {
         // matrix 0 with coefficient 1 at every node: the instance's
         // waveform gives generator 0 the base 1 and no segments (the
         // folded constants); wave-uniform, read once
