@@ -146,18 +146,14 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
     double g[SLOTS];
     g[0] = v;
     xbuf[lane] = v;
+    // The store, then every LDS read, before any VALU work of the product
+    // (its DPP moves and multiplies, the caller's accumulation of the
+    // previous term): the round trip starts as soon as v is ready and the
+    // reads queue right behind the store.
+    __builtin_amdgcn_sched_barrier(0);
     wave_exchange();
-    if (SLOTS > 1)
-        g[1] = (NPAIR > 0 || partner) ? swap_pair(v) : xbuf[col[1]];
     const double2 *x2 = reinterpret_cast<const double2 *>(xbuf);
-    if constexpr (NPAIR == 1) {
-        // the quad layout: the other pair of the quad is lanes
-        // (r & ~1) ^ 2 and its odd neighbour, in reach of DPP.  With the
-        // unrolled Taylor chain the four moves ran 1 % faster than the
-        // 16-byte read (with the Taylor loop, 6 % slower).
-        g[2] = quad_perm<0x0A>(v);  // [2, 2, 0, 0]
-        g[3] = quad_perm<0x5F>(v);  // [3, 3, 1, 1]
-    } else {
+    if constexpr (NPAIR != 1) {
 #pragma unroll
         for (int k = 0; k < NPAIR; ++k) {
             const double2 o = x2[col[2 + 2 * k] >> 1];
@@ -165,9 +161,22 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
             g[3 + 2 * k] = o.y;
         }
     }
+    if (SLOTS > 1 && !(NPAIR > 0 || partner))
+        g[1] = xbuf[col[1]];
 #pragma unroll
     for (int sl = 2 + 2 * NPAIR; sl < SLOTS; ++sl)
         g[sl] = xbuf[col[sl]];
+    __builtin_amdgcn_sched_barrier(0);
+    if (SLOTS > 1 && (NPAIR > 0 || partner))
+        g[1] = swap_pair(v);
+    if constexpr (NPAIR == 1) {
+        // the quad layout: the other pair of the quad is lanes
+        // (r & ~1) ^ 2 and its odd neighbour, in reach of DPP.  With the
+        // unrolled Taylor chain the four moves ran 1 % faster than the
+        // 16-byte read (with the Taylor loop, 6 % slower).
+        g[2] = quad_perm<0x0A>(v);  // [2, 2, 0, 0]
+        g[3] = quad_perm<0x5F>(v);  // [3, 3, 1, 1]
+    }
     wave_exchange();
     double y = a[0] * g[0];
 #pragma unroll
@@ -320,8 +329,21 @@ __device__ __forceinline__ double scalar_inv_factorial()
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// 1 / T! as an operand: 1 / 1! and 1 / 2! stay plain constants (inline
+// operands), the others are built where they are used.
+template <int T>
+__device__ __forceinline__ double inv_factorial_operand()
+{
+    if constexpr (T <= 2)
+        return inv_factorial(T);
+    else
+        return scalar_inv_factorial<T>();
+}
+
 // Terms T..M of taylor_action's unrolled chain, stopping after term m
-// (template recursion, so every 1 / T! is a compile-time constant).
+// (template recursion, so every 1 / T! is a compile-time constant).  On
+// entry `term` is y_(T-1), not yet added to `acc` when T > 1: it is added
+// after the product that consumes it has issued its LDS store.
 template <int SLOTS, int NPAIR, int M, int T = 1>
 __device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
                                              const int (&col)[SLOTS],
@@ -330,18 +352,20 @@ __device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
                                              bool partner, int m)
 {
     if constexpr (T <= M) {
-        if (T > m)
+        if (T > m) {
+            if constexpr (T > 1)
+                acc = fma(term, inv_factorial_operand<T - 1>(), acc);
             return;
-        // 1 / 1! and 1 / 2! stay plain constants (inline operands)
-        double inv;
-        if constexpr (T <= 2)
-            inv = inv_factorial(T);
-        else
-            inv = scalar_inv_factorial<T>();
-        term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane, partner);
-        acc = fma(term, inv, acc);
+        }
+        const double y =
+            ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane, partner);
+        if constexpr (T > 1)
+            acc = fma(term, inv_factorial_operand<T - 1>(), acc);
+        term = y;
         taylor_chain<SLOTS, NPAIR, M, T + 1>(a, col, term, acc, xbuf, lane,
                                              partner, m);
+    } else {
+        acc = fma(term, inv_factorial_operand<M>(), acc);
     }
 }
 
@@ -357,7 +381,9 @@ __device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
 // M = 0: a loop over the runtime degree m, 1 / t! broadcast from lane t of
 // inv_lane (which holds 1 / lane!).  All three give the same doubles in
 // the same order; the unrolled forms avoid the loop's v_readlane, whose
-// SGPR result stalls the dependent chain.
+// SGPR result stalls the dependent chain.  Each term is added to the sum
+// after the next product has stored it to LDS (same operations, same
+// order), so the accumulation is off the chain's critical path.
 template <int SLOTS, int NPAIR, int M>
 __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
                                                 const int (&col)[SLOTS],
@@ -370,10 +396,13 @@ __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
     if constexpr (M > 0) {
 #pragma unroll
         for (int t = 1; t <= M; ++t) {
-            term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
-                                           partner);
-            acc = fma(term, inv_factorial(t), acc);
+            const double y = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
+                                                     lane, partner);
+            if (t > 1)
+                acc = fma(term, inv_factorial(t - 1), acc);
+            term = y;
         }
+        acc = fma(term, inv_factorial(M), acc);
     } else if constexpr (M < 0) {
         // The degree through an opaque scalar move: the per-term exit tests
         // then stay scalar compares inside the chain, where the compiler
@@ -385,10 +414,13 @@ __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
                                        partner, mm);
     } else {
         for (int t = 1; t <= m; ++t) {
-            term = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
-                                           partner);
-            acc = fma(term, read_lane(inv_lane, t), acc);
+            const double y = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
+                                                     lane, partner);
+            if (t > 1)
+                acc = fma(term, read_lane(inv_lane, t - 1), acc);
+            term = y;
         }
+        acc = fma(term, read_lane(inv_lane, m), acc);
     }
     return acc;
 }
