@@ -18,6 +18,10 @@
 // at A in direction E: the powers of A (forward terms w_t) and of A^T
 // (adjoint terms u_i) are shared between the exponential and its
 // derivative.  mu is then mapped through T_{m_a}(A)^T = sum_i u_i.
+// Both chains carry unscaled powers, v_i = (A^T)^i mu and z_t = A^t y, so no
+// multiply by 1 / i sits on either chain: Abar = sum_i v_i q_i^T with
+// q_i = sum_t z_t / (i + t + 1)! (c_it u_i w_t^T = v_i z_t^T / (i + t + 1)!)
+// and mu <- sum_i v_i / i!.
 //
 // Work split: one workgroup of two wavefronts per instance.  Wavefront 0
 // (the recompute wavefront) re-runs segment p - 1 of ckpt_every steps from
@@ -29,8 +33,8 @@
 // current one: two independent dependency chains per lane.
 //
 // Abar is accumulated on a transposed ELL pattern (lane c, slot -> row r):
-// entry Abar[r][c] = sum_i u_i[r] q_i[c] with q_i = sum_t c_it w_t, so the
-// gathers of u_i that the product A^T u_i needs also feed the accumulation.
+// entry Abar[r][c] = sum_i v_i[r] q_i[c], so the gathers of v_i that the
+// product A^T v_i needs also feed the accumulation.
 
 #include "ssmx_common.h"
 
@@ -137,6 +141,23 @@ __device__ __forceinline__ double amp_seek(const AdjParams &p,
     return damp;
 }
 
+// q_i = sum_{t < MA - i} z_t / (i + t + 1)! from the unscaled forward
+// terms z_t = A^t y (every weight a compile-time constant); the stored-input
+// and the recomputing wavefronts both form it here, in this order.
+template <int MA>
+__device__ __forceinline__ void forward_weights(const double (&z)[MA],
+                                                double (&q)[MA])
+{
+#pragma unroll
+    for (int i = 0; i < MA; ++i) {
+        double qv = ssmx::inv_factorial(i + 1) * z[0];
+#pragma unroll
+        for (int t = 1; t + i < MA; ++t)
+            qv = fma(ssmx::inv_factorial(i + t + 1), z[t], qv);
+        q[i] = qv;
+    }
+}
+
 // y = A v on an ELL pattern (slot 0 = own lane, slot 1 = partner lane
 // r ^ 1 by DPP or a gathered column, slots 2.. gathered through `xbuf`).
 // Same order of operations as the propagator's product.
@@ -149,12 +170,17 @@ __device__ __forceinline__ double ell_product(const double (&a)[SL],
     double g[SL];
     g[0] = v;
     xbuf[lane] = v;
+    // the store and the gathers before any VALU work of the product
+    __builtin_amdgcn_sched_barrier(0);
     wave_exchange();
-    if (SL > 1)
-        g[1] = use_partner ? swap_pair(v) : xbuf[col[1]];
+    if (SL > 1 && !use_partner)
+        g[1] = xbuf[col[1]];
 #pragma unroll
     for (int sl = 2; sl < SL; ++sl)
         g[sl] = xbuf[col[sl]];
+    __builtin_amdgcn_sched_barrier(0);
+    if (SL > 1 && use_partner)
+        g[1] = swap_pair(v);
     wave_exchange();
     double y = a[0] * g[0];
 #pragma unroll
@@ -191,12 +217,12 @@ __device__ __forceinline__ void assemble(double (&a)[SL], const double *lv,
 // Two wavefronts per instance (one workgroup of 128 threads):
 //  wavefront 0 re-runs segment p - 1 from its checkpoint, storing every
 //    exponential's input in the workspace, and in the same loop evaluates
-//    the forward terms w_t = A^t y / t! of segment p's exponentials in
-//    reverse order (both are chains of products on the forward pattern, so
-//    one 16-byte exchange per term serves both); it reduces them to
-//    q_i = sum_t c_it w_t and hands q over through a double-buffered LDS
-//    slot;
-//  wavefront 1 runs the adjoint terms u_i of segment p in reverse order and
+//    the forward terms z_t = A^t y of segment p's exponentials in reverse
+//    order (both are chains of products on the forward pattern, so one
+//    16-byte exchange per term serves both); it reduces them to
+//    q_i = sum_t z_t / (i + t + 1)! (forward_weights) and hands q over
+//    through a ring of LDS slots;
+//  wavefront 1 runs the adjoint terms v_i of segment p in reverse order and
 //    accumulates the exponent gradient on the transposed pattern.
 // q travels through a ring of kRing LDS slots with two counters (produced,
 // consumed): wavefront 0 may run up to kRing exponentials ahead of
@@ -388,25 +414,14 @@ adjoint_kernel(AdjParams p)
                         ypre = ypre2;
                         next(pk, pj);
                         ypre2 = yin[((size_t)pk * J + pj) * kWave + lane];
-                        // w_t = A^t y / t!, one 8-byte exchange per term
+                        // z_t = A^t y (unscaled), one 8-byte exchange
+                        // per term
 #pragma unroll
                         for (int t = 1; t < MA; ++t)
                             W[t] = ell_product<SF>(aV, cf, W[t - 1], x0,
-                                                   lane, use_partner) *
-                                   (1.0 / t);
-                        // q_i = sum_t c_it w_t
+                                                   lane, use_partner);
                         double Q[MA];
-#pragma unroll
-                        for (int i = 0; i < MA; ++i) {
-                            double c = 1.0 / (i + 1);
-                            double qv = c * W[0];
-#pragma unroll
-                            for (int t = 1; t + i < MA; ++t) {
-                                c = c * t / (i + t + 1);
-                                qv = fma(c, W[t], qv);
-                            }
-                            Q[i] = qv;
-                        }
+                        forward_weights<MA>(W, Q);
                         // wait for a free slot, fill it, publish it
                         for (long poll = 0;
                              !broken &&
@@ -574,7 +589,7 @@ adjoint_kernel(AdjParams p)
                     term = y;
                     if (t <= m_fwd)
                         acc = fma(term, ssmx::inv_factorial(t), acc);
-                    W[t] = v * (1.0 / t);
+                    W[t] = v;
                 }
                 // the rest of the recomputation's chain: for the common
                 // degree with constant 1 / t! (a v_readlane result would
@@ -604,20 +619,8 @@ adjoint_kernel(AdjParams p)
                     }
                 }
                 if (v_on) {
-                    // q_i = sum_t c_it w_t, c_i0 = 1 / (i + 1),
-                    // c_it = c_i,t-1 * t / (i + t + 1)
                     double Q[MA];
-#pragma unroll
-                    for (int i = 0; i < MA; ++i) {
-                        double c = 1.0 / (i + 1);
-                        double qv = c * W[0];
-#pragma unroll
-                        for (int t = 1; t + i < MA; ++t) {
-                            c = c * t / (i + t + 1);
-                            qv = fma(c, W[t], qv);
-                        }
-                        Q[i] = qv;
-                    }
+                    forward_weights<MA>(W, Q);
                     // wait for a free slot, fill it, publish it
                     for (long poll = 0;
                          !broken &&
@@ -788,7 +791,7 @@ adjoint_kernel(AdjParams p)
                     __hip_atomic_store(consumed, (int)taken, __ATOMIC_RELEASE,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
 
-                double u = lam;
+                double u = lam;           // v_0 = mu
                 double lam_new = lam;
                 double T[ST];
 #pragma unroll
@@ -800,13 +803,19 @@ adjoint_kernel(AdjParams p)
 #pragma unroll
                 for (int i = 0; i < MA; ++i) {
                     x1[lane] = u;
+                    // the store and the gathers before any VALU work
+                    __builtin_amdgcn_sched_barrier(0);
                     ssmx::wave_exchange();
                     double gu[ST];
                     gu[0] = u;
-                    gu[1] = partner_slot ? swap_pair(u) : x1[ct[1]];
+                    if (!partner_slot)
+                        gu[1] = x1[ct[1]];
 #pragma unroll
                     for (int sl = 2; sl < ST; ++sl)
                         gu[sl] = x1[ct[sl]];
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (partner_slot)
+                        gu[1] = swap_pair(u);
                     // The previous term's accumulation, issued while this
                     // term's gathers are in flight (it needs none of them).
                     // The empty asm keeps each accumulation here: T is only
@@ -829,8 +838,9 @@ adjoint_kernel(AdjParams p)
                         else
                             u0 = fma(a_t[sl], gu[sl], u0);
                     }
-                    u = (u0 + u1) * (1.0 / (i + 1));
-                    lam_new += u;
+                    // v_(i+1) = A^T v_i, unscaled; mu_new += v / (i+1)!
+                    u = u0 + u1;
+                    lam_new = fma(u, ssmx::inv_factorial(i + 1), lam_new);
 #pragma unroll
                     for (int sl = 0; sl < ST; ++sl)
                         gp[sl] = gu[sl];
