@@ -127,8 +127,8 @@ struct RevArgs {
 
 // Step I (1..MA) of the transposed chain on buffer (I - 1) & 1:
 // u <- (A^T u) / I, lambda += u, v_t += c_It u for t < MA - I.  The
-// compile-time 1.0 / I is the same double as kInvW[I] and needs no
-// v_readlane (whose SGPR result stalls the next VALU instruction).
+// compile-time 1.0 / I needs no v_readlane (whose SGPR result stalls the
+// next VALU instruction).
 template <int SF, int WT, int MA, int I>
 __device__ __forceinline__ void u_chain(double &u, double &acc,
                                         double (&v)[MA],
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kWave *kMaxWaves) void adjoint_wide_kernel(
     for (int o = 0; o < d.n_obs; ++o)
         srow = fma(p.seed[(size_t)inst * d.n_obs + o], p.obs[o * NP + row],
                    srow);
-    const double inv_lane = kInvW[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+    const double inv_lane = kInvW.v[lane <= SSMX_MAX_TAYLOR ? lane : 0];
 
     double B[SE];
 #pragma unroll

@@ -162,7 +162,7 @@ void wide_kernel(
         gp[sl] = xb0 + ((sl < d.slots ? p.ell_col[sl * NP + row] : row) &
                         (kBuf - 1));
     double *own = xb0 + row;
-    const double inv_lane = kInvW[lane <= SSMX_MAX_TAYLOR ? lane : 0];
+    const double inv_lane = kInvW.v[lane <= SSMX_MAX_TAYLOR ? lane : 0];
 
     const int inst_c = active ? inst : 0;     // idle slots read instance 0
     const double *coef_i =

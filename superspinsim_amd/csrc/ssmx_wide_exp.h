@@ -26,14 +26,8 @@ constexpr int kMaxBlockWaves = 16;
 // holds up to 512 rows; 4 KiB fits a ds_read immediate offset
 constexpr int kBuf = SSMX_MAX_N_WIDE;
 
-__constant__ double kInvW[SSMX_MAX_TAYLOR + 1] = {
-    0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,
-    1.0 / 6,    1.0 / 7,    1.0 / 8,  1.0 / 9,  1.0 / 10, 1.0 / 11,
-    1.0 / 12,   1.0 / 13,   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
-    1.0 / 18,   1.0 / 19,   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23,
-    1.0 / 24,   1.0 / 25,   1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
-    1.0 / 30,   1.0 / 31,   1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35,
-    1.0 / 36,   1.0 / 37,   1.0 / 38, 1.0 / 39, 1.0 / 40};
+// 1 / q! for the Taylor loop, q = 0..40 (lane q reads entry q).
+__constant__ ssmx::InvFactorials kInvW = ssmx::make_inv_factorials();
 
 // Wave-uniform inputs of one exponential.
 struct ExpArgs {
@@ -47,20 +41,27 @@ struct ExpArgs {
 // y = A v for this thread's row, reading buffer P (0 or 1): v is written to
 // the row's own entry, then WS slots are combined (slot 0 diagonal, slot 1
 // partner, others gathered).  Two accumulation chains (even and odd
-// slots), as in ssmx_propagate.hip.
+// slots).  The store, the barrier and the gathers come before any VALU work
+// of the product (the partner's DPP moves, the caller's accumulation of the
+// previous term).
 template <int SLOTS, int WS, int P>
 __device__ __forceinline__ double wide_product(
     const double (&a)[WS], double v, double *own,
     const double *const (&gp)[SLOTS], bool partner)
 {
     own[P * kBuf] = v;
+    __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
     double g[WS];
     g[0] = v;
-    g[1] = partner ? swap_pair(v) : gp[1][P * kBuf];
+    if (!partner)
+        g[1] = gp[1][P * kBuf];
 #pragma unroll
     for (int sl = 2; sl < WS; ++sl)
         g[sl] = gp[sl][P * kBuf];
+    __builtin_amdgcn_sched_barrier(0);
+    if (partner)
+        g[1] = swap_pair(v);
     double y0 = a[0] * g[0];
     double y1 = a[1] * g[1];
 #pragma unroll
@@ -73,9 +74,11 @@ __device__ __forceinline__ double wide_product(
     return y0 + y1;
 }
 
-// Taylor terms t = T..M of one exponential, unrolled: term <- (A term) / t,
-// acc += term, product t on buffer (P0 + t - 1) & 1, 1 / t constants (the
-// same doubles as kInvW, so the same bits as the loop in wide_exp).
+// Taylor terms t = T..M of one exponential, unrolled: y_t = A y_(t-1)
+// (unscaled powers) on buffer (P0 + t - 1) & 1 and acc <- fma(y_t, 1 / t!,
+// acc), added after the next product has stored y_t (the same operations in
+// the same order as the loop in wide_exp, so the same bits).  On entry
+// `term` is y_(T-1), not yet added when T > 1.
 template <int SLOTS, int WS, int M, int P0, int T = 1>
 __device__ __forceinline__ void wide_chain(const double (&a)[WS],
                                            double &term, double &acc,
@@ -84,21 +87,21 @@ __device__ __forceinline__ void wide_chain(const double (&a)[WS],
                                            bool partner)
 {
     if constexpr (T <= M) {
-        {
-#pragma clang fp contract(off)
-            term = wide_product<SLOTS, WS, (P0 + T - 1) & 1>(a, term, own,
-                                                            gp, partner) *
-                   (1.0 / T);
-            acc += term;
-        }
+        const double y = wide_product<SLOTS, WS, (P0 + T - 1) & 1>(
+            a, term, own, gp, partner);
+        if constexpr (T > 1)
+            acc = fma(term, ssmx::inv_factorial(T - 1), acc);
+        term = y;
         wide_chain<SLOTS, WS, M, P0, T + 1>(a, term, acc, own, gp, partner);
+    } else {
+        acc = fma(term, ssmx::inv_factorial(M), acc);
     }
 }
 
 // x <- exp(A_j) x for this thread's row: assemble the row of A_j from the
 // staged values (slots < WS), then s substeps of the degree-m Taylor
 // action.  `parity` is the buffer the next product writes.  With UNROLL,
-// degree kUnrolledTaylor runs unrolled (wide_chain): a v_readlane of 1 / t
+// degree kUnrolledTaylor runs unrolled (wide_chain): a v_readlane of 1 / t!
 // would stall every product.  Both forms give the same bits.
 template <int SLOTS, int WS, bool UNROLL>
 __device__ __forceinline__ double wide_exp(double x, const ExpArgs &e,
@@ -133,11 +136,10 @@ __device__ __forceinline__ double wide_exp(double x, const ExpArgs &e,
         for (int sl = 0; sl < WS; ++sl)
             a[sl] *= e.inv_sub;
     }
-    // term_t = (A term_{t-1}) / t; acc += term_t
+    // y_t = A y_(t-1) unscaled; acc <- fma(y_t, 1 / t!, acc)
     auto taylor_term = [&](double y, int t, double &term, double &acc) {
-#pragma clang fp contract(off)
-        term = y * read_lane(inv_lane, t);
-        acc += term;
+        term = y;
+        acc = fma(y, read_lane(inv_lane, t), acc);
     };
     for (int sub = 0; sub < e.substeps; ++sub) {
         double term = x;
