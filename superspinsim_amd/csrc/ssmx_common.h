@@ -134,12 +134,18 @@ __device__ __forceinline__ int fetched_pairs(const int (&col)[SLOTS],
 // one fma chain in slot order (the register-held slots first, the LDS
 // gathers last, so one fma follows the last gather), and the rounding
 // depends on the row's values and columns only, not on how the components
-// were fetched.
-template <int SLOTS, int NPAIR>
+// were fetched.  `pre` runs after the gathers are issued and before the
+// arithmetic: a caller may fill a[] there.
+// No work between a product's gathers and its arithmetic.
+struct NoPrologue {
+    __device__ void operator()() const {}
+};
+
+template <int SLOTS, int NPAIR, typename Pre = NoPrologue>
 __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
                                             const int (&col)[SLOTS],
                                             double v, double *xbuf, int lane,
-                                            bool partner)
+                                            bool partner, Pre &&pre = Pre{})
 {
     static_assert(NPAIR >= 0 && 2 + 2 * NPAIR <= (SLOTS > 2 ? SLOTS : 2),
                   "fetched pairs exceed the slots");
@@ -167,6 +173,9 @@ __device__ __forceinline__ double ell_apply(const double (&a)[SLOTS],
     for (int sl = 2 + 2 * NPAIR; sl < SLOTS; ++sl)
         g[sl] = xbuf[col[sl]];
     __builtin_amdgcn_sched_barrier(0);
+    // work the caller puts behind the round trip (the first product of an
+    // exponential assembles the exponent's row here: a[] is read below)
+    pre();
     if (SLOTS > 1 && (NPAIR > 0 || partner))
         g[1] = swap_pair(v);
     if constexpr (NPAIR == 1) {
@@ -359,8 +368,10 @@ __device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
         }
         const double y =
             ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane, partner);
-        if constexpr (T > 1)
+        if constexpr (T > 1) {
             acc = fma(term, inv_factorial_operand<T - 1>(), acc);
+            asm volatile("" : "+v"(acc));  // not sunk to the chain's end
+        }
         term = y;
         taylor_chain<SLOTS, NPAIR, M, T + 1>(a, col, term, acc, xbuf, lane,
                                              partner, m);
@@ -384,26 +395,36 @@ __device__ __forceinline__ void taylor_chain(const double (&a)[SLOTS],
 // SGPR result stalls the dependent chain.  Each term is added to the sum
 // after the next product has stored it to LDS (same operations, same
 // order), so the accumulation is off the chain's critical path.
-template <int SLOTS, int NPAIR, int M>
+template <int SLOTS, int NPAIR, int M, typename Pre = NoPrologue>
 __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
                                                 const int (&col)[SLOTS],
                                                 double x, double *xbuf,
                                                 int lane, bool partner, int m,
-                                                double inv_lane)
+                                                double inv_lane,
+                                                Pre &&pre = Pre{})
 {
     double term = x;
     double acc = x;
     if constexpr (M > 0) {
+        // `pre` (which may fill a[]) runs inside the first product, behind
+        // the LDS store of x and its gathers
+        const double y1 = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf, lane,
+                                                  partner, pre);
+        term = y1;
 #pragma unroll
-        for (int t = 1; t <= M; ++t) {
+        for (int t = 2; t <= M; ++t) {
             const double y = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
                                                      lane, partner);
-            if (t > 1)
-                acc = fma(term, inv_factorial(t - 1), acc);
+            acc = fma(term, inv_factorial(t - 1), acc);
+            // each term is added inside the product after it; without the
+            // empty asm the compiler sinks all M additions to the end of
+            // the chain, holding every term in registers until then
+            asm volatile("" : "+v"(acc));
             term = y;
         }
         acc = fma(term, inv_factorial(M), acc);
     } else if constexpr (M < 0) {
+        pre();
         // The degree through an opaque scalar move: the per-term exit tests
         // then stay scalar compares inside the chain, where the compiler
         // would otherwise hoist all -M of them out of the step loop as lane
@@ -413,6 +434,7 @@ __device__ __forceinline__ double taylor_action(const double (&a)[SLOTS],
         taylor_chain<SLOTS, NPAIR, -M>(a, col, term, acc, xbuf, lane,
                                        partner, mm);
     } else {
+        pre();
         for (int t = 1; t <= m; ++t) {
             const double y = ell_apply<SLOTS, NPAIR>(a, col, term, xbuf,
                                                      lane, partner);

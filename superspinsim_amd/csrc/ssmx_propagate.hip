@@ -160,10 +160,11 @@ __device__ __forceinline__ double unit_weight_sum(const double *w, int j,
 // columns.  NPAIR: leading slot pairs fetched whole (ell_apply).  MG > 0:
 // the MG (= M) generator rows are read from LDS once into registers;
 // MG < 0: the first -MG rows are, the others are read at each exponent.
-// UNIT0 (MG = 2, two exponentials, lab frame, waveform mode): matrix 0's
-// coefficient is 1 at every node (the folded constant generators), so its
-// share of each exponent, fma(f0_j, G0, 0), is formed once per launch; f0_j
-// is computed as step_factor computes it, so the row is the same bits.  The
+// UNIT0 (MG = 2, two exponentials, one substep, lab frame, waveform mode):
+// matrix 0's coefficient is 1 at every node (the folded constant
+// generators), so its share of each exponent, fma(f0_j, G0, 0), is formed
+// once per launch; f0_j is computed as step_factor computes it, so the row
+// is the same bits.  The
 // caller checks that f0_0 == f0_1 (CF4:2's two weight rows hold the same
 // two numbers, and their sums are equal), so one row serves both
 // exponentials.
@@ -254,7 +255,10 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
     int cursor = 0;
     double factor = 0.0;
 
-    const double inv_sub = 1.0 / (double)d.substeps;
+    // UNIT0 is dispatched with one substep only (no scaling of the row,
+    // no substep loop in the step)
+    const int substeps = UNIT0 ? 1 : d.substeps;
+    const double inv_sub = 1.0 / (double)substeps;
     double xint = 0.0;
     const double half_h = 0.5 * d.h;
 
@@ -290,7 +294,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
             const double x_prev = x;
             // exponential j with its row a assembled
             auto exponential = [&](int j, double (&a)[SLOTS]) {
-                if (d.substeps > 1) {
+                if (substeps > 1) {
                     keep_branch();
 #pragma unroll
                     for (int sl = 0; sl < SLOTS; ++sl)
@@ -304,20 +308,32 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                                 lane] = x;
                 }
                 // the adjoint's recomputation repeats these roundings
-                for (int sub = 0; sub < d.substeps; ++sub)
+                for (int sub = 0; sub < substeps; ++sub)
                     x = taylor_action<SLOTS, NPAIR, TAY>(
                         a, col, x, xbuf, lane, use_partner, d.taylor,
                         inv_lane);
             };
             if constexpr (UNIT0) {
+                // one substep: the row of A_j is assembled inside the first
+                // product, behind the LDS store of x and its gathers
 #pragma unroll
                 for (int jj = 1; jj >= 0; --jj) {
                     double a[SLOTS];
-                    const double f = read_lane(factor, (b * 2 + jj) * 2 + 1);
+                    auto assemble = [&]() {
+                        const double f =
+                            read_lane(factor, (b * 2 + jj) * 2 + 1);
 #pragma unroll
-                    for (int sl = 0; sl < SLOTS; ++sl)
-                        a[sl] = fma(f, g[1][sl], cst[sl]);
-                    exponential(jj, a);
+                        for (int sl = 0; sl < SLOTS; ++sl)
+                            a[sl] = fma(f, g[1][sl], cst[sl]);
+                    };
+                    if constexpr (STORE)
+                        p.state_out[(((size_t)inst * total + step) * J +
+                                     jj) *
+                                        kWave +
+                                    lane] = x;
+                    x = taylor_action<SLOTS, NPAIR, TAY>(
+                        a, col, x, xbuf, lane, use_partner, d.taylor,
+                        inv_lane, assemble);
                 }
             } else {
                 for (int j = J - 1; j >= 0; --j) {
@@ -480,7 +496,8 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         // waveform gives generator 0 the base 1 and no segments (the
         // folded constants); wave-uniform, read once
         if (!rotating && (d.flags & SSMX_COEF_WAVE) && d.n_exp == 2 &&
-            p.waves[inst].seg_count[0] == 0 && p.waves[inst].base[0] == 1.0 &&
+            d.substeps == 1 && p.waves[inst].seg_count[0] == 0 &&
+            p.waves[inst].base[0] == 1.0 &&
             unit_weight_sum(p.w, 0, d.n_node) ==
                 unit_weight_sum(p.w, 1, d.n_node)) {
             dispatch_layout<SLOTS, CHAIN, MG, STORE, true>(p, lds, xbuf, M,
