@@ -89,8 +89,10 @@ __device__ __forceinline__ void wide_chain(const double (&a)[WS],
     if constexpr (T <= M) {
         const double y = wide_product<SLOTS, WS, (P0 + T - 1) & 1>(
             a, term, own, gp, partner);
-        if constexpr (T > 1)
+        if constexpr (T > 1) {
             acc = fma(term, ssmx::inv_factorial(T - 1), acc);
+            asm volatile("" : "+v"(acc));  // not sunk to the chain's end
+        }
         term = y;
         wide_chain<SLOTS, WS, M, P0, T + 1>(a, term, acc, own, gp, partner);
     } else {
