@@ -662,6 +662,9 @@ adjoint_kernel(AdjParams p)
     }
 
     // ------------- adjoint terms -------------
+    // The adjoint chain bounds the kernel (the forward-term wavefront has
+    // slack): when both are ready to issue, this one goes first.
+    __builtin_amdgcn_s_setprio(2);
     int ct[ST];
     ct[0] = lane;
 #pragma unroll
