@@ -246,12 +246,15 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
     };
 
     // Lane-parallel coefficient batches: entry e = (b * J + j) * M + m.
-    const int JM = J * M;
+    // UNIT0 reads matrix 1's factors only (matrix 0's is the constant row):
+    // entry e = b * J + j, twice the steps per batch
+    const int Mb = UNIT0 ? 1 : M;
+    const int JM = J * Mb;
     const int B = kWave / JM;  // steps per batch (validated >= 1)
     const long total = (long)d.n_out * d.fine;
     const int my_b = lane / JM;
-    const int my_j = (lane % JM) / M;
-    const int my_m = lane % M;
+    const int my_j = (lane % JM) / Mb;
+    const int my_m = UNIT0 ? 1 : lane % M;
     int cursor = 0;
     double factor = 0.0;
 
@@ -321,7 +324,7 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                     double a[SLOTS];
                     auto assemble = [&]() {
                         const double f =
-                            read_lane(factor, (b * 2 + jj) * 2 + 1);
+                            read_lane(factor, b * 2 + jj);
 #pragma unroll
                         for (int sl = 0; sl < SLOTS; ++sl)
                             a[sl] = fma(f, g[1][sl], cst[sl]);
