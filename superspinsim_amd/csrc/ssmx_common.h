@@ -289,10 +289,6 @@ __device__ __forceinline__ void assemble_row_mix(double (&a)[SLOTS],
     }
 }
 
-// 1.0 / T (correctly rounded, as the compiler folds it) built by two
-// scalar moves at the point of use: as plain constants the compiler hoists
-// all of them out of the step loop, where they spill and come back through
-// v_readlane, whose SGPR result stalls the next multiply.
 // 1 / t! in double, the same value wherever it is formed: the product
 // 2 * 3 * ... * t rounded in that order (exact up to t = 18), then 1 / it.
 __host__ __device__ constexpr double inv_factorial(int t)
@@ -317,17 +313,10 @@ __host__ __device__ constexpr InvFactorials make_inv_factorials()
     return r;
 }
 
-template <int T>
-__device__ __forceinline__ double scalar_inverse()
-{
-    constexpr uint64_t bits = __builtin_bit_cast(uint64_t, 1.0 / T);
-    uint32_t lo, hi;
-    asm volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"((uint32_t)bits));
-    asm volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"((uint32_t)(bits >> 32)));
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-// inv_factorial(T) built where it is used, as scalar_inverse builds 1 / T.
+// inv_factorial(T) built by two scalar moves at the point of use: as plain
+// constants the compiler hoists all of them out of the step loop, where
+// they spill and come back through v_readlane, whose SGPR result stalls the
+// next multiply.
 template <int T>
 __device__ __forceinline__ double scalar_inv_factorial()
 {

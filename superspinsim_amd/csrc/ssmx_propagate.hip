@@ -160,7 +160,8 @@ __device__ __forceinline__ double unit_weight_sum(const double *w, int j,
 // columns.  NPAIR: leading slot pairs fetched whole (ell_apply).  MG > 0:
 // the MG (= M) generator rows are read from LDS once into registers;
 // MG < 0: the first -MG rows are, the others are read at each exponent.
-// UNIT0 (MG = 2, two exponentials, one substep, lab frame, waveform mode):
+// UNIT0 (MG = 2 or -2, two exponentials, one substep, lab frame, waveform
+// mode):
 // matrix 0's coefficient is 1 at every node (the folded constant
 // generators), so its share of each exponent, fma(f0_j, G0, 0), is formed
 // once per launch; f0_j is computed as step_factor computes it, so the row
@@ -201,7 +202,8 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                                     2 +
                                 (sl & 1)];
     }
-    static_assert(!UNIT0 || MG == 2, "UNIT0 needs MG = 2");
+    static_assert(!UNIT0 || MG == 2 || MG == -2,
+                  "UNIT0 needs matrices 0 and 1 in registers");
     double cst[SLOTS];
     if constexpr (UNIT0) {
         const double f0 = d.h * unit_weight_sum(p.w, 0, d.n_node);
@@ -246,15 +248,16 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
     };
 
     // Lane-parallel coefficient batches: entry e = (b * J + j) * M + m.
-    // UNIT0 reads matrix 1's factors only (matrix 0's is the constant row):
-    // entry e = b * J + j, twice the steps per batch
-    const int Mb = UNIT0 ? 1 : M;
+    // UNIT0 reads the factors of matrices 1..M-1 only (matrix 0's share is
+    // the constant row): entry e = (b * J + j) * (M - 1) + m - 1, more steps
+    // per batch
+    const int Mb = UNIT0 ? (MG == 2 ? 1 : M - 1) : M;
     const int JM = J * Mb;
     const int B = kWave / JM;  // steps per batch (validated >= 1)
     const long total = (long)d.n_out * d.fine;
     const int my_b = lane / JM;
     const int my_j = (lane % JM) / Mb;
-    const int my_m = UNIT0 ? 1 : lane % M;
+    const int my_m = UNIT0 ? 1 + lane % Mb : lane % M;
     int cursor = 0;
     double factor = 0.0;
 
@@ -323,11 +326,31 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                 for (int jj = 1; jj >= 0; --jj) {
                     double a[SLOTS];
                     auto assemble = [&]() {
-                        const double f =
-                            read_lane(factor, b * 2 + jj);
+                        const int base = (b * 2 + jj) * Mb;
+                        const double f = read_lane(factor, base);
 #pragma unroll
                         for (int sl = 0; sl < SLOTS; ++sl)
                             a[sl] = fma(f, g[1][sl], cst[sl]);
+                        if constexpr (MG < 0) {
+                            // matrices 2..M-1 from the staged values, in
+                            // assemble_row_mix's order
+                            for (int m = 2; m < M; ++m) {
+                                const double fm =
+                                    read_lane(factor, base + m - 1);
+                                const double2 *v =
+                                    reinterpret_cast<const double2 *>(
+                                        vals + m * SLOTP * kWave) +
+                                    lane;
+#pragma unroll
+                                for (int pr = 0; pr < SLOTP / 2; ++pr) {
+                                    const double2 vv = v[pr * kWave];
+                                    a[2 * pr] = fma(fm, vv.x, a[2 * pr]);
+                                    if (2 * pr + 1 < SLOTS)
+                                        a[2 * pr + 1] =
+                                            fma(fm, vv.y, a[2 * pr + 1]);
+                                }
+                            }
+                        }
                     };
                     if constexpr (STORE)
                         p.state_out[(((size_t)inst * total + step) * J +
@@ -494,7 +517,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void propagate_kernel(
         load_columns<SLOTS>(p, lane, col);
         np = fetched_pairs(col, lane, d.partner != 0);
     }
-    if constexpr (MG == 2 && CHAIN > 0) {
+    if constexpr ((MG == 2 || MG == -2) && CHAIN > 0) {
         // matrix 0 with coefficient 1 at every node: the instance's
         // waveform gives generator 0 the base 1 and no segments (the
         // folded constants); wave-uniform, read once
