@@ -447,6 +447,17 @@ constexpr int kUnrolledTaylor = 11;
 // kernels.
 constexpr int kUnrolledMax = 16;
 
+// sum_s w[j,s] c(t_s) for a coefficient c = 1 at every node, in the order
+// and roundings of the kernels' step factors (fma(w, c, cw) over s).
+__device__ __forceinline__ double unit_weight_sum(const double *w, int j,
+                                                  int n_node)
+{
+    double cw = 0.0;
+    for (int s = 0; s < n_node; ++s)
+        cw = fma(w[j * n_node + s], 1.0, cw);
+    return cw;
+}
+
 // Node time of global output interval i, fine division r, node c, in the
 // reference's order without contraction:
 //   ((t0 + (r dt) / F) + dt i) + (dt c) / F

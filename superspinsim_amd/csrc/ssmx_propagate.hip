@@ -56,6 +56,7 @@ using ssmx::fetched_pairs;
 using ssmx::read_lane;
 using ssmx::set_error;
 using ssmx::taylor_action;
+using ssmx::unit_weight_sum;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
 using ssmx::wave_sum;
@@ -142,17 +143,6 @@ __device__ __forceinline__ void load_columns(const Params &p, int lane,
     for (int s = 1; s < SLOTS; ++s)
         col[s] = s < p.d.slots ? p.ell_col[s * kWave + lane]
                                : pad_column(s, lane);
-}
-
-// step_factor's sum_s w[j,s] c(t_s) for a coefficient c = 1 at every node,
-// in step_factor's order and roundings.
-__device__ __forceinline__ double unit_weight_sum(const double *w, int j,
-                                                  int n_node)
-{
-    double cw = 0.0;
-    for (int s = 0; s < n_node; ++s)
-        cw = fma(w[j * n_node + s], 1.0, cw);
-    return cw;
 }
 
 // One instance on one wavefront, after the kernel has staged the generator

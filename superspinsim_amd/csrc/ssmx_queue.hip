@@ -17,6 +17,7 @@
 #include "ssmx_common.h"
 
 #include <mutex>
+#include <type_traits>
 
 namespace {
 
@@ -33,6 +34,7 @@ using ssmx::fetched_pairs;
 using ssmx::read_lane;
 using ssmx::set_error;
 using ssmx::taylor_action;
+using ssmx::unit_weight_sum;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
 using ssmx::wave_sum;
@@ -122,27 +124,43 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
     }
     const double obs0 = p.obs[lane];
     const double obs1 = d.n_obs > 1 ? p.obs[kWave + lane] : 0.0;
-    const int JM = J * M;
-    const int B = kWave / JM;
-    const int my_b = lane / JM;
-    const int my_j = (lane % JM) / M;
-    const int my_m = lane % M;
     const double inv_sub = 1.0 / (double)d.substeps;
     const double half_h = 0.5 * d.h;
 
-    // Every pass takes one queue entry; the counter only grows, so every
-    // wavefront leaves the loop once the n_inst entries are handed out.
-    for (;;) {
-        int q = 0;
-        if (lane == 0)
-            q = atomicAdd(p.counter, 1);
-        q = __builtin_amdgcn_readfirstlane(q);
-        if (q >= d.n_inst)
-            break;
-        const int inst = p.order[q];
+    // Matrix 0 with coefficient 1 at every node (the folded constant
+    // generators: the protocol's waveform gives generator 0 the base 1 and
+    // no segments): its share of both exponents, fma(f0, G0, 0), is formed
+    // once per wavefront when the two weight sums are equal (CF4:2), as in
+    // ssmx_propagate.  f0 is the step factor's value, so the rows are the
+    // same bits.
+    constexpr bool kUnitPath = MG == 2 && TAY > 0;
+    double cst[SLOTS];
+    bool unit_ok = false;
+    if constexpr (kUnitPath) {
+        if (!rotating && J == 2 && d.substeps == 1 &&
+            unit_weight_sum(p.w, 0, d.n_node) ==
+                unit_weight_sum(p.w, 1, d.n_node)) {
+            unit_ok = true;
+            const double f0 = d.h * unit_weight_sum(p.w, 0, d.n_node);
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; ++sl)
+                cst[sl] = fma(f0, g[0][sl], 0.0);
+        }
+    }
+
+    // One protocol.  UNIT: the constant-row path (one substep, factors of
+    // matrices 1..M-1 only in the coefficient batch, each exponent's row
+    // assembled inside its first product); otherwise the general path.
+    auto run = [&](auto unit_tag, int inst, const ssmx_wave *wv) {
+        constexpr bool UNIT = decltype(unit_tag)::value;
+        const int Mb = UNIT ? M - 1 : M;
+        const int JM = J * Mb;
+        const int B = kWave / JM;
+        const int my_b = lane / JM;
+        const int my_j = (lane % JM) / Mb;
+        const int my_m = UNIT ? 1 + lane % Mb : lane % M;
         const long total = p.steps[inst];
         const long k_from = p.read_from[inst];
-        const ssmx_wave *wv = p.waves + inst;
         double x = lane < n ? p.rho0[(size_t)inst * n + lane] : 0.0;
         double xint = 0.0;
         int cursor = 0;
@@ -158,24 +176,60 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
                              : 0.0;
             }
             const double x_prev = x;
-            for (int j = J - 1; j >= 0; --j) {
-                double a[SLOTS];
-                if constexpr (MG > 0)
-                    assemble_row_mix<SLOTS, MG>(a, g, vals, factor,
-                                                (b * J + j) * M, M, lane);
-                else
-                    assemble_row(a, vals, factor, (b * J + j) * M, M, lane);
-                if (d.substeps > 1) {
-                    keep_branch();
+            if constexpr (UNIT) {
 #pragma unroll
-                    for (int sl = 0; sl < SLOTS; ++sl)
-                        a[sl] *= inv_sub;
-                }
-                // the adjoint's recomputation repeats these roundings
-                for (int sub = 0; sub < d.substeps; ++sub)
+                for (int jj = 1; jj >= 0; --jj) {
+                    double a[SLOTS];
+                    auto assemble = [&]() {
+                        const int base = (b * 2 + jj) * Mb;
+                        const double f = read_lane(factor, base);
+#pragma unroll
+                        for (int sl = 0; sl < SLOTS; ++sl)
+                            a[sl] = fma(f, g[1][sl], cst[sl]);
+                        // matrices 2..M-1 from the staged values, in
+                        // assemble_row_mix's order
+                        constexpr int SLOTP = SLOTS + (SLOTS & 1);
+                        for (int m = 2; m < M; ++m) {
+                            const double fm = read_lane(factor, base + m - 1);
+                            const double2 *v =
+                                reinterpret_cast<const double2 *>(
+                                    vals + m * SLOTP * kWave) +
+                                lane;
+#pragma unroll
+                            for (int pr = 0; pr < SLOTP / 2; ++pr) {
+                                const double2 vv = v[pr * kWave];
+                                a[2 * pr] = fma(fm, vv.x, a[2 * pr]);
+                                if (2 * pr + 1 < SLOTS)
+                                    a[2 * pr + 1] =
+                                        fma(fm, vv.y, a[2 * pr + 1]);
+                            }
+                        }
+                    };
                     x = taylor_action<SLOTS, NPAIR, TAY>(
                         a, col, x, xbuf, lane, use_partner, d.taylor,
-                        inv_lane);
+                        inv_lane, assemble);
+                }
+            } else {
+                for (int j = J - 1; j >= 0; --j) {
+                    double a[SLOTS];
+                    if constexpr (MG > 0)
+                        assemble_row_mix<SLOTS, MG>(a, g, vals, factor,
+                                                    (b * J + j) * M, M, lane);
+                    else
+                        assemble_row(a, vals, factor, (b * J + j) * M, M,
+                                     lane);
+                    if (d.substeps > 1) {
+                        keep_branch();
+#pragma unroll
+                        for (int sl = 0; sl < SLOTS; ++sl)
+                            a[sl] *= inv_sub;
+                    }
+                    // the adjoint's recomputation repeats these roundings
+                    for (int sub = 0; sub < d.substeps; ++sub)
+                        x = taylor_action<SLOTS, NPAIR, TAY>(
+                            a, col, x, xbuf, lane, use_partner, d.taylor,
+                            inv_lane);
+                }
             }
             if (rotating) {
                 const double xp = swap_pair(x);
@@ -192,6 +246,26 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
             if (lane == 0)
                 p.out[(size_t)inst * d.n_obs + 1] = v1;
         }
+    };
+
+    // Every pass takes one queue entry; the counter only grows, so every
+    // wavefront leaves the loop once the n_inst entries are handed out.
+    for (;;) {
+        int q = 0;
+        if (lane == 0)
+            q = atomicAdd(p.counter, 1);
+        q = __builtin_amdgcn_readfirstlane(q);
+        if (q >= d.n_inst)
+            break;
+        const int inst = p.order[q];
+        const ssmx_wave *wv = p.waves + inst;
+        if constexpr (kUnitPath) {
+            if (unit_ok && wv->seg_count[0] == 0 && wv->base[0] == 1.0) {
+                run(std::true_type{}, inst, wv);
+                continue;
+            }
+        }
+        run(std::false_type{}, inst, wv);
     }
 }
 
