@@ -49,7 +49,7 @@ PEAK_HBM_GBS = 8000.0
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB copy
 # (fetch x2.0, write x1.0); the summary is kept with the profiles.
 MEASURED_TRAFFIC = {
-    ("odmr7-nv-n49", 4096, 99900): (1322.2 * 2.0 + 35238.3 * 1.0) * 1024,
+    ("odmr7-nv-n49", 4096, 99900): (1320.9 * 2.0 + 35238.0 * 1.0) * 1024,
 }
 
 
