@@ -34,7 +34,7 @@ import ctypes
 import numpy as np
 
 from superspinsim_amd import _lib
-from superspinsim_amd.propagator import _ptr, _torch, device_tensor
+from superspinsim_amd.propagator import _bank_slots, _ptr, _torch, device_tensor
 from superspinsim_amd.sweep import BatchSimulator, waveform_records
 
 _f64 = np.float64
@@ -85,6 +85,49 @@ def shared_adjoint_ell(fwd_col, mats, dmats):
     for r in range(n):
         col[0, r] = 2 + len(rows[r])
     return col, val_f, val_t, dval_t
+
+
+def transposed_adjoint_ell(mats, dmats):
+    """The adjoint chain's own ELL pattern (n <= 64).
+
+    Row r lists r (slot 0), its partner r ^ 1 (slot 1, exchanged by DPP)
+    and then every other column c with A[c][r] or D[c][r] non-zero, in an
+    order chosen against LDS bank conflicts (``_bank_slots``: the chain
+    gathers these columns one at a time, and the forward kernel's slot
+    order, which the forward terms must keep, does not constrain it).  A
+    padding slot may repeat another column of the row; each column carries
+    its values once.  Returns (col_t, val_t, dval_t) with
+    val_t[m, s, r] = G_m[c][r] and dval_t[t, s, r] = D_t[c][r] for
+    c = col_t[s, r].
+    """
+    M, n, _ = mats.shape
+    tr = np.any(mats != 0, axis=0).T
+    if dmats.shape[0]:
+        tr = tr | np.any(dmats != 0, axis=0).T
+    entries = [[int(c) for c in np.nonzero(tr[r])[0] if c not in (r, r ^ 1)]
+               if r < n else [] for r in range(64)]
+    G = max(len(e) for e in entries)
+    slots = 2 + G
+    col = np.empty((slots, 64), dtype=np.int32)
+    lanes = np.arange(64, dtype=np.int32)
+    col[0] = lanes
+    col[1] = lanes ^ 1
+    if G:
+        col[2:] = _bank_slots(entries, G, lambda r: r, 16, 16).T
+    val_t = np.zeros((M, slots, 64))
+    dval_t = np.zeros((dmats.shape[0], slots, 64))
+    for r in range(n):
+        seen = set()
+        for sl in range(slots):
+            c = int(col[sl, r])
+            if c >= n or c in seen:
+                continue
+            seen.add(c)
+            val_t[:, sl, r] = mats[:, c, r]
+            if dmats.shape[0]:
+                dval_t[:, sl, r] = dmats[:, c, r]
+    col[0, :] = slots
+    return col, val_t, dval_t
 
 
 def wide_adjoint_ell(ell, mats, dmats):
@@ -323,20 +366,20 @@ class GradientProblem:
                 slots=col_e.shape[0], slots_t=col_t.shape[0],
                 slots_f=slots_f)
         else:
-            col, val_f, val_t, dval_t = shared_adjoint_ell(prop.ell.col, mp,
-                                                           dp)
-            if col.shape[0] > _lib.ADJ_MAX_SLOTS:
+            col, val_f, _, _ = shared_adjoint_ell(prop.ell.col, mp, dp)
+            # the adjoint chain gathers its own, bank-ordered pattern
+            col_t, val_t, dval_t = transposed_adjoint_ell(mp, dp)
+            if max(col.shape[0], col_t.shape[0]) > _lib.ADJ_MAX_SLOTS:
                 raise NotImplementedError(
-                    f"adjoint pattern of {col.shape[0]} slots > "
-                    f"{_lib.ADJ_MAX_SLOTS}")
-            d_col = device_tensor(col)
+                    f"adjoint pattern of {col.shape[0]} / {col_t.shape[0]} "
+                    f"slots > {_lib.ADJ_MAX_SLOTS}")
             tables = dict(
-                d_col=d_col,
+                d_col=device_tensor(col),
                 # forward values are zero beyond the forward kernel's slots
                 d_val_f=device_tensor(val_f[:, :prop.ell.slots].copy()),
-                d_col_t=d_col, d_val_t=device_tensor(val_t),
+                d_col_t=device_tensor(col_t), d_val_t=device_tensor(val_t),
                 d_dval_t=device_tensor(dval_t) if dconst.shape[0] else None,
-                slots=prop.ell.slots, slots_t=col.shape[0], slots_f=0)
+                slots=prop.ell.slots, slots_t=col_t.shape[0], slots_f=0)
         state = dict(
             theta=theta, amplitudes=amplitudes, prop=prop, program=program,
             derivatives=bool(derivatives),

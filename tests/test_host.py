@@ -348,6 +348,54 @@ def test_wide_adjoint_tables():
             assert len(hits) == 1
 
 
+def test_transposed_adjoint_tables():
+    """The adjoint chain's own pattern holds A^T and D^T exactly once per
+    entry, keeps slot 1 the partner, and needs fewer modelled LDS passes
+    than the shared forward/transposed pattern."""
+    from superspinsim_amd.adjoint import (shared_adjoint_ell,
+                                          transposed_adjoint_ell)
+    rng = np.random.default_rng(7)
+    n = 49
+    pat = rng.random((n, n)) < 0.08
+    pat |= np.eye(n, dtype=bool)
+    mats = np.stack([pat * rng.normal(size=(n, n)),
+                     (rng.random((n, n)) < 0.04) * rng.normal(size=(n, n))])
+    dmats = np.stack([(rng.random((n, n)) < 0.03) * rng.normal(size=(n, n))
+                      for _ in range(3)])
+    col_t, val_t, dval_t = transposed_adjoint_ell(mats, dmats)
+    lanes = np.arange(64)
+    assert np.array_equal(col_t[1], lanes ^ 1)
+    At = np.zeros((2, 64, 64))
+    Dt = np.zeros((3, 64, 64))
+    for r in range(64):
+        for sl in range(col_t.shape[0]):
+            c = r if sl == 0 else int(col_t[sl, r])
+            At[:, r, c] += val_t[:, sl, r]
+            Dt[:, r, c] += dval_t[:, sl, r]
+    assert np.array_equal(At[:, :n, :n], mats.transpose(0, 2, 1))
+    assert np.array_equal(Dt[:, :n, :n], dmats.transpose(0, 2, 1))
+    assert not At[:, n:].any() and not At[:, :, n:].any()
+
+    def passes(col):
+        tot = 0
+        for sl in range(2, col.shape[0]):
+            for g in range(0, 64, 16):
+                per = {}
+                for a in {int(col[sl, r]) for r in range(g, g + 16)}:
+                    per[a % 16] = per.get(a % 16, 0) + 1
+                tot += max(per.values())
+        return tot
+
+    fwd = np.tile(lanes, (7, 1)).astype(np.int32)
+    fwd[1] = lanes ^ 1
+    for r in range(n):
+        cols = [c for c in np.nonzero(pat[r])[0] if c not in (r, r ^ 1)][:5]
+        fwd[2:2 + len(cols), r] = cols
+        fwd[0, r] = 2 + len(cols)
+    shared = shared_adjoint_ell(fwd, mats, dmats)[0]
+    assert passes(col_t) <= passes(shared)
+
+
 @pytest.mark.parametrize("norm", [0.01, 0.19, 0.8, 3.0, 40.0])
 def test_taylor_plan_meets_tolerance(norm):
     m, s = taylor_plan(norm, tol=1e-16)
