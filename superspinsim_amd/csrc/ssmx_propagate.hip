@@ -59,6 +59,7 @@ using ssmx::taylor_action;
 using ssmx::unit_weight_sum;
 using ssmx::swap_pair;
 using ssmx::wave_coef;
+using ssmx::wave_max;
 using ssmx::wave_sum;
 
 constexpr int kWavesPerBlock = 8;
@@ -181,16 +182,44 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
     const bool use_partner = d.partner != 0;
 
     constexpr int MGR = MG < 0 ? -MG : MG;
+    // UNIT0 with three or more matrices: when each row of the wavefront is
+    // non-zero in at most one of matrices 1..M-1 (a packed system's copies
+    // of a varying generator, one per member block), the lane's own matrix
+    // 1 + beta is the one held in g[1], and the exponent row is
+    // fma(f_(1+beta), G_(1+beta), cst): the other matrices would add exact
+    // zeros to it.  Wave-uniform; found once from the staged values.
+    int beta = 0;
+    bool disjoint = false;
+    if constexpr (UNIT0 && MG < 0) {
+        constexpr int SLOTP = SLOTS + (SLOTS & 1);
+        int planes = 0;
+        for (int m = 1; m < M; ++m) {
+            bool nz = false;
+#pragma unroll
+            for (int sl = 0; sl < SLOTS; ++sl)
+                nz |= vals[((m * (SLOTP / 2) + sl / 2) * kWave + lane) * 2 +
+                           (sl & 1)] != 0.0;
+            if (nz) {
+                beta = m - 1;
+                ++planes;
+            }
+        }
+        disjoint = wave_max(planes) <= 1;
+        if (!disjoint)
+            beta = 0;
+    }
     double g[MGR > 0 ? MGR : 1][SLOTS];
     if constexpr (MGR > 0) {
         constexpr int SLOTP = SLOTS + (SLOTS & 1);
 #pragma unroll
-        for (int m = 0; m < MGR; ++m)
+        for (int m = 0; m < MGR; ++m) {
+            const int plane = m == 1 ? 1 + beta : m;
 #pragma unroll
             for (int sl = 0; sl < SLOTS; ++sl)
-                g[m][sl] = vals[((m * (SLOTP / 2) + sl / 2) * kWave + lane) *
-                                    2 +
-                                (sl & 1)];
+                g[m][sl] =
+                    vals[((plane * (SLOTP / 2) + sl / 2) * kWave + lane) * 2 +
+                         (sl & 1)];
+        }
     }
     static_assert(!UNIT0 || MG == 2 || MG == -2,
                   "UNIT0 needs matrices 0 and 1 in registers");
@@ -317,6 +346,17 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                     double a[SLOTS];
                     auto assemble = [&]() {
                         const int base = (b * 2 + jj) * Mb;
+                        if constexpr (MG < 0) {
+                            if (disjoint) {
+                                // the lane's block factor, across lanes
+                                const double f =
+                                    __shfl(factor, base + beta, kWave);
+#pragma unroll
+                                for (int sl = 0; sl < SLOTS; ++sl)
+                                    a[sl] = fma(f, g[1][sl], cst[sl]);
+                                return;
+                            }
+                        }
                         const double f = read_lane(factor, base);
 #pragma unroll
                         for (int sl = 0; sl < SLOTS; ++sl)
