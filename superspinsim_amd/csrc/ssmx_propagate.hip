@@ -363,10 +363,14 @@ __device__ __forceinline__ void propagate_instance(const Params &p,
                             a[sl] = fma(f, g[1][sl], cst[sl]);
                         if constexpr (MG < 0) {
                             // matrices 2..M-1 from the staged values, in
-                            // assemble_row_mix's order
+                            // assemble_row_mix's order; a factor of 0 (the
+                            // generator off in this step) would add exact
+                            // zeros (the row is never -0)
                             for (int m = 2; m < M; ++m) {
                                 const double fm =
                                     read_lane(factor, base + m - 1);
+                                if (fm == 0.0)
+                                    continue;
                                 const double2 *v =
                                     reinterpret_cast<const double2 *>(
                                         vals + m * SLOTP * kWave) +

@@ -187,10 +187,15 @@ __device__ __forceinline__ void queue_loop(const QParams &p,
                         for (int sl = 0; sl < SLOTS; ++sl)
                             a[sl] = fma(f, g[1][sl], cst[sl]);
                         // matrices 2..M-1 from the staged values, in
-                        // assemble_row_mix's order
+                        // assemble_row_mix's order; a matrix whose factor
+                        // is 0 (a generator switched off in this step, such
+                        // as the readout's excitation before its window)
+                        // would add exact zeros (the row is never -0)
                         constexpr int SLOTP = SLOTS + (SLOTS & 1);
                         for (int m = 2; m < M; ++m) {
                             const double fm = read_lane(factor, base + m - 1);
+                            if (fm == 0.0)
+                                continue;
                             const double2 *v =
                                 reinterpret_cast<const double2 *>(
                                     vals + m * SLOTP * kWave) +
