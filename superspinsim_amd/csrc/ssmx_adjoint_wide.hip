@@ -5,8 +5,8 @@
 // Same quantities as ssmx_adjoint.hip (include/ssmx.h): per instance the
 // adjoint state lambda_0 = dLoss/d rho0 and
 //   grad_theta[t] = sum_{k,j} f_{kj,theta_gen} <Abar_kj, D_t>,
-//   Abar = sum_{i + t < MA} c_it u_i w_t^T,  c_it = i! t! / (i + t + 1)!,
-//   u_i = (A^T)^i mu / i!,  w_t = A^t y / t!,
+//   Abar = sum_{i + t < MA} u_i w_t^T / (i + t + 1)!,
+//   u_i = (A^T)^i mu,  w_t = A^t y  (unscaled powers),
 // for each exponential exp(A) with input y and output cotangent mu.
 //
 // One workgroup of W = ceil(n / 64) wavefronts per instance, thread r
@@ -16,8 +16,9 @@
 //     exponential (ssmx_wide_exp.h, bit-identical states), storing the
 //     input of every exponential in the workspace;
 //  2. the reverse sweep: per exponential (last applied first) the chain
-//     u_i on the transposed pattern (lambda <- exp(A)^T lambda = sum u_i,
-//     and v_t = sum_i c_it u_i, row-local), then the chain w_t on the
+//     u_i on the transposed pattern (lambda <- exp(A)^T lambda =
+//     sum u_i / i!, and v_t = sum_i u_i / (i + t + 1)!, row-local), then
+//     the chain w_t on the
 //     forward pattern extended by the derivative tables' non-zeros, where
 //     each exchange of w_t also adds f_theta v_t[r] w_t[c] to the row's
 //     accumulator B[r][c]; finally lambda += w_k s (readout weight).
@@ -33,6 +34,7 @@
 
 namespace {
 
+using ssmx::inv_factorial;
 using ssmx::node_time;
 using ssmx::set_error;
 using ssmx::wave_coef_seek;
@@ -63,15 +65,6 @@ struct AWParams {
     double *grad_theta;
     double *lambda_out;
 };
-
-// i! t! / (i + t + 1)! = 1 / ((i + t + 1) binom(i + t, i))
-__host__ __device__ constexpr double frechet_coef(int i, int t)
-{
-    double b = 1.0;
-    for (int k = 1; k <= i; ++k)
-        b = b * (double)(t + k) / (double)k;
-    return 1.0 / ((double)(i + t + 1) * b);
-}
 
 // h sum_s w[j,s] c_m(t_s) of propagator step `step` in the propagator's
 // rounding order (non-rotating frame), for steps visited in any order.
@@ -125,10 +118,11 @@ struct RevArgs {
     double *own;        // the row's entry in exchange buffer 0
 };
 
-// Step I (1..MA) of the transposed chain on buffer (I - 1) & 1:
-// u <- (A^T u) / I, lambda += u, v_t += c_It u for t < MA - I.  The
-// compile-time 1.0 / I needs no v_readlane (whose SGPR result stalls the
-// next VALU instruction).
+// Step I (1..MA) of the transposed chain on buffer (I - 1) & 1: the
+// unscaled power u <- A^T u, lambda += u / I!, v_t += u / (I + t + 1)! for
+// t < MA - I.  The factorials are compile-time constants (no v_readlane,
+// whose SGPR result stalls the next VALU instruction), and no multiply by
+// 1 / I sits in the chain.
 template <int SF, int WT, int MA, int I>
 __device__ __forceinline__ void u_chain(double &u, double &acc,
                                         double (&v)[MA],
@@ -136,19 +130,18 @@ __device__ __forceinline__ void u_chain(double &u, double &acc,
                                         const RevArgs &r,
                                         const double *const (&gpt)[SF])
 {
-    u = wide_product<SF, WT, (I - 1) & 1>(at, u, r.own, gpt, true) *
-        (1.0 / I);
-    acc += u;
+    u = wide_product<SF, WT, (I - 1) & 1>(at, u, r.own, gpt, true);
+    acc = fma(u, inv_factorial(I), acc);
 #pragma unroll
     for (int t = 0; t < MA - I; ++t)
-        v[t] = fma(frechet_coef(I, t), u, v[t]);
+        v[t] = fma(inv_factorial(I + t + 1), u, v[t]);
     if constexpr (I < MA)
         u_chain<SF, WT, MA, I + 1>(u, acc, v, at, r, gpt);
 }
 
 // Exchange T (0..MA-1) of the forward-term chain on buffer (MA + T) & 1:
 // B += f_theta v_T w[c] over the WE extended slots, then
-// w <- (A w) / (T + 1) over the WA matrix slots (not after the last).
+// w <- A w (unscaled) over the WA matrix slots (not after the last).
 template <int SE, int WE, int WA, int MA, int T>
 __device__ __forceinline__ void w_chain(double &w, const double (&v)[MA],
                                         double (&B)[SE],
@@ -179,7 +172,7 @@ __device__ __forceinline__ void w_chain(double &w, const double (&v)[MA],
             else
                 y0 = fma(a[sl], g[sl], y0);
         }
-        w = (y0 + y1) * (1.0 / (T + 1));
+        w = y0 + y1;
         w_chain<SE, WE, WA, MA, T + 1>(w, v, B, a, r, gpe);
     }
 }
@@ -212,7 +205,7 @@ __device__ __forceinline__ void reverse_u(double &lam, double (&v)[MA],
     double acc = lam;
 #pragma unroll
     for (int t = 0; t < MA; ++t)
-        v[t] = frechet_coef(0, t) * u;
+        v[t] = inv_factorial(t + 1) * u;
     u_chain<SF, WT, MA, 1>(u, acc, v, at, r, gpt);
     lam = acc;
 }
